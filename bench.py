@@ -1,0 +1,144 @@
+"""Benchmark: RK4 time stepping of the 513x513x9 Gaussian-shoal tank (6th order, 5-wavelength
+sponges, 64-component JONSWAP initial surface), mixed precision (FP32 multigrid preconditioner,
+FP64 outer solve / matvec / RK4 state) against the plain FP64 GPU form, on one MI355X.
+
+Prints one JSON line.  value = DOF-steps/s of the mixed-precision run (DOF = Nx*Ny*(Nz+1), the
+reference's count, OceanWave3D.f90:29).  The timed region covers exactly --steps RK4 steps with the
+state resident in HBM, bracketed by device synchronisation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "oceanwave3d-fortran90_amd"))
+
+WORKLOAD = "shoal513"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_tank(ow3d, problems, mg_precision, solver=1):
+    cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD)
+    cfg = dict(cfg)
+    cfg.update(mg_precision=mg_precision, solver=solver, reltol=1e-8, abstol=0.0, gmres_m=23)
+    T = ow3d.Tank(cfg, **bath)
+    for z in zones:
+        T.add_relax_zone(*z)
+    T.set_state(E, P)
+    return T, cfg, dt
+
+
+def run_steps(T, steps, dt, t0=0.0):
+    lat = []
+    for s in range(steps):
+        a = time.perf_counter()
+        T.rk4_steps(1, t0 + s * dt, dt)
+        lat.append(time.perf_counter() - a)
+    return lat
+
+
+def cpu_baseline(cfg_gpu, budget_s=25.0):
+    """The oracle in reference-faithful mode (CSR multigrid with lexicographic Gauss-Seidel, unfused
+    derivative passes, GMRES(23)), one thread, on RK4 steps of the same tank family."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    from ow3d import problems
+    oracle.build()
+    cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD)
+    p = dict(nx=cfg["nx"], ny=cfg["ny"], nz=cfg["nz"], alpha=3, beta=3, gamma=3, gridz=1, lx=cfg["lx"], ly=cfg["ly"],
+             lz=1.0, g=9.81, nonlinear=1, solver=1, precond=3, gmres_m=23, reltol=1e-8, abstol=0.0, maxit=1, nu1=1,
+             nu2=1, mg_maxgrids=20, extrapolation=1, mg_mode=0, cd_quirk=1)
+    T = oracle.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+    for z in zones:
+        T.add_relax_zone(*z)
+    T.set_state(E, P)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        T.rk4_step(steps * dt, dt)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 3:
+            break
+    el = time.perf_counter() - t0
+    dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
+    T.close()
+    return dict(value=dof * steps / el, unit="DOF-steps/s", cores=1, kind="port",
+                sample=f"{steps} RK4 step(s) of the {cfg['nx']}x{cfg['ny']}x{cfg['nz']} shoal tank, {el:.1f} s on 1 host core")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+    import ow3d
+    from ow3d import problems
+
+    results = {}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        T, cfg, dt = make_tank(ow3d, problems, prec)
+        run_steps(T, args.warmup, dt)
+        T.set_state(*make_tank_state(problems))
+        lat = run_steps(T, args.steps, dt)
+        E, Pp, W, phi = T.get_state()
+        tot, solves, itmin, itmax = T.stats()
+        results[label] = dict(lat=lat, E=E, P=Pp, iters=(tot, solves, itmin, itmax))
+        if label == "mixed":
+            T.time_matvec(5)
+            mv_ms = T.time_matvec(200)
+            ncol = T.nxg * T.nyg
+            b_mv = 16 * T.n + 8 * 5 * ncol
+        T.close()
+    dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
+    mix, f64 = results["mixed"], results["fp64"]
+    t_mix = sum(mix["lat"])
+    t_f64 = sum(f64["lat"])
+    err_eta = float(np.max(np.abs(mix["E"] - f64["E"])) / np.max(np.abs(f64["E"])))
+    err_phi = float(np.max(np.abs(mix["P"] - f64["P"])) / np.max(np.abs(f64["P"])))
+    achieved = b_mv / (mv_ms * 1e-3) / 1e9
+    out = {
+        "metric": "DOF-steps/s mixed vs FP64 (<=1e-6 rel eta err); p50/p99 RK4-step latency",
+        "value": dof * args.steps / t_mix,
+        "unit": "DOF-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mix / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 multigrid preconditioner)",
+        "data": "synthetic",
+        "config": {"workload": "513x513x9 Gaussian shoal, 6th order, 5-wavelength sponges, JONSWAP seed 0",
+                   "dof": dof, "solver": "GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid"},
+        "fp64": {"value": dof * args.steps / t_f64, "ms_per_step": 1e3 * t_f64 / args.steps,
+                 "p50_ms": 1e3 * float(np.percentile(f64["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(f64["lat"], 99)),
+                 "iters": f64["iters"]},
+        "mixed": {"p50_ms": 1e3 * float(np.percentile(mix["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(mix["lat"], 99)),
+                  "iters": mix["iters"], "rel_err_eta_vs_fp64": err_eta, "rel_err_phi_vs_fp64": err_phi},
+        "speedup_mixed_vs_fp64": t_f64 / t_mix,
+        "roofline": {"bound": "hbm", "kernel": "k_matvec3 (high-order sigma-Laplace matvec)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": mv_ms * 1e3, "algorithmic_bytes": b_mv},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out), flush=True)
+
+
+def make_tank_state(problems):
+    cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD)
+    return E, P
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,108 @@
+/*
+ * ow3d.h -- C ABI of the MI355X (gfx950) implementation of OceanWave3D's
+ * Laplace-solve-per-RK4-stage hot path.
+ *
+ * Arrays are host arrays owned by the caller unless the function name ends in
+ * _device.  Layout is the reference's: 3D fields are sigma-fastest,
+ *     phi[k + nzg*(i + nxg*j)],  nxg = Nx+2, nyg = Ny+2 (1 when Ny == 1), nzg = Nz+1,
+ * 2D free-surface fields are E[i + nxg*j].  All values are FP64.
+ *
+ * Return codes: 0 ok, >0 the iterative solver did not reach its tolerance
+ * (the reference ignores this, iterative_solution.f90:115-134; here it is reported),
+ * <0 error (ow3d_last_error() gives the message).
+ *
+ * One handle per host thread; each handle owns its device buffers and its HIP stream.
+ */
+#ifndef OW3D_H
+#define OW3D_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ow3d_tank ow3d_tank;
+
+/* Mirrors the reference's input-deck knobs for the path (ReadInputFileParameters.f90):
+ * line 3 (grid), line 4 (half widths), line 7 (solver), plus the precision choice. */
+typedef struct ow3d_config {
+    int nx, ny, nz;            /* Nx, Ny, Nz (deck line 3); Ny == 1 selects the x-z problem */
+    int alpha, beta, gamma;    /* half widths of the high-order stencils (alpha == beta == gamma) */
+    int gridz;                 /* 0 even sigma levels, 1 sine-stretched (GridZ) */
+    double lx, ly, lz;         /* domain lengths; lz used only for gridz == 0 */
+    double g;                  /* gravitational acceleration */
+    int nonlinear;             /* 1 fully nonlinear free-surface conditions, 0 linear */
+    int solver;                /* 0 defect correction (dc.f), 1 GMRES */
+    int precond;               /* 0 none, 3 geometric multigrid */
+    int gmres_m;               /* GMRESmaxiterations: Krylov dimension and matvec budget */
+    double reltol, abstol;     /* stopping test on the preconditioned residual */
+    int maxit;                 /* multigrid cycles per preconditioner application */
+    int nu1, nu2;              /* pre / post smoothing sweeps */
+    int mg_maxgrids;           /* cap on multigrid levels (<= 0: no cap) */
+    int extrapolation;         /* extrapolationONOFF: initial guesses from earlier solutions */
+    int mg_precision;          /* 0 FP64 multigrid, 1 FP32 multigrid, 2 FP32 + FP16 storage on the two coarsest levels */
+} ow3d_config;
+
+/* Replaces the set-up the reference does in OceanWave3DT0Setup.f90:32-409 for this path
+ * (SetupCompDomain, PreProcessDiffStencils, MGPreProcess): grids, stencils and the multigrid
+ * hierarchy.  h, hx, hy, hxx, hyy are plane arrays (nxg*nyg); hx.. may be NULL (zero). */
+int  ow3d_create(const ow3d_config *cfg, const double *h, const double *hx, const double *hy,
+                 const double *hxx, const double *hyy, ow3d_tank **out);
+void ow3d_destroy(ow3d_tank *t);
+int  ow3d_dims(const ow3d_tank *t, int *nxg, int *nyg, int *nzg, int *mg_levels);
+const char *ow3d_last_error(void);
+
+/* Replaces BuildLinearSystem(Nx,Ny,Nz,PHI,output,GridStruct,alpha,beta,gamma)
+ * (functions/BuildLinearSystem.f90:1-18): y = A phi with the sigma coefficients of the
+ * surface last given to ow3d_set_surface. */
+int ow3d_laplace_apply(ow3d_tank *t, const double *phi, double *y);
+
+/* Replaces MultiGridSolveLinearSystem(r,u,Nx,Ny,Nz,...) (multigrid/MultiGridSolveLinearSystem.f90:1-15):
+ * u = M^-1 r, maxit multigrid V cycles from u = 0. */
+int ow3d_precond_apply(ow3d_tank *t, const double *r, double *u);
+
+/* Replaces iterative_solution(rhss,n_rows,A_times_x,sol,CurrentGrid)
+ * (wrappers/iterative_solution.f90:1-16) with A_times_x = BuildLinearSystem: solves A sol = rhs
+ * from the initial guess in sol.  iters = matrix-vector products, res = final preconditioned
+ * residual norm (estimate for GMRES). */
+int ow3d_solve(ow3d_tank *t, const double *rhs, double *sol, int *iters, double *res);
+
+/* Replaces DifferentiationsFreeSurfacePlane(Wavefield,...) (timeintegration/DifferentiationsFreeSurfacePlane.f90:1-32)
+ * followed by DetermineTransformationConstantsArray (functions/DetermineTransformationConstantsArray.f90):
+ * updates the wall ghost values of E and P in place and sets the operator's sigma coefficients.
+ * Derivative outputs may be NULL. */
+int ow3d_set_surface(ow3d_tank *t, double *E, double *P, double *Ex, double *Exx, double *Ey,
+                     double *Eyy, double *Px, double *Py);
+
+/* Replaces VerticalFreeSurfaceVelocity(W,Nx,Ny,Nz,sol,DiffStencils,dsdz,gamma)
+ * (functions/VerticalFreeSurfaceVelocity.f90:1-20). */
+int ow3d_vertical_velocity(ow3d_tank *t, const double *phi, double *W);
+
+/* Replaces rhsFreeSurface3D(t,Wavefield,g,rhsE,rhsP,Nx,Ny) (main/rhsFreeSurface3D.f90:1-29) and
+ * rhsLinearFreeSurface3D (main/rhsLinearFreeSurface3D.f90:1-13), using the plane derivatives of the
+ * surface last given to ow3d_set_surface. */
+int ow3d_rhs_free_surface(ow3d_tank *t, const double *E, const double *W, double *rhsE, double *rhsP);
+
+/* Damping (sponge) zone of RelaxationModule_new.f90 (WavegenONOFF = 0): E, P <- gam*E, gam*P over
+ * plane indices [i0,i1] x [j0,j1] (0-based, inclusive); gam runs along x (axis 0) or y (axis 1). */
+int ow3d_add_relax_zone(ow3d_tank *t, int axis, int i0, int i1, int j0, int j1, const double *gam);
+
+/* Device-resident time stepping: replaces Runge_Kutta_4(rhsFreeSurface) (timeintegration/Runge_Kutta_4.f90:1-299)
+ * driven by OceanWave3DTakeATimeStep.  State is copied in once, stepped on the device, copied out on demand. */
+int ow3d_set_state(ow3d_tank *t, const double *E, const double *P, const double *W, const double *phi);
+int ow3d_get_state(ow3d_tank *t, double *E, double *P, double *W, double *phi);
+int ow3d_rk4_steps(ow3d_tank *t, int nsteps, double time0, double dt);
+/* iteration statistics as in iterative_solution.f90:117-134: total, solves, min, max */
+int ow3d_stats(const ow3d_tank *t, long long *total, long long *solves, int *itmin, int *itmax);
+
+/* Device-pointer form of ow3d_laplace_apply for callers that keep phi resident (stream: hipStream_t
+ * or NULL for the handle's stream). */
+int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, double *d_y, void *stream);
+/* The handle's HIP stream. */
+void *ow3d_stream(ow3d_tank *t);
+/* Average duration (ms) of the high-order matvec kernel over the last n_launches launches timed with
+ * HIP events on the handle's stream (ow3d_time_matvec runs them). */
+int ow3d_time_matvec(ow3d_tank *t, int n_launches, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

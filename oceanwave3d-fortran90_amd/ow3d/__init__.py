@@ -1,0 +1,206 @@
+"""Python host interface to libow3d.so, the MI355X (gfx950) implementation of OceanWave3D's
+Laplace-solve-per-RK4-stage path.
+
+The entry points mirror the reference subroutines they replace (see include/ow3d.h):
+
+    Tank.laplace_apply   BuildLinearSystem            (functions/BuildLinearSystem.f90)
+    Tank.precond_apply   MultiGridSolveLinearSystem   (multigrid/MultiGridSolveLinearSystem.f90)
+    Tank.solve           iterative_solution           (wrappers/iterative_solution.f90)
+    Tank.set_surface     DifferentiationsFreeSurfacePlane + DetermineTransformationConstantsArray
+    Tank.vertical_velocity  VerticalFreeSurfaceVelocity
+    Tank.rhs_free_surface   rhsFreeSurface3D / rhsLinearFreeSurface3D
+    Tank.rk4_steps       Runge_Kutta_4 driven by OceanWave3DTakeATimeStep
+
+Arrays follow the reference layout: 3D fields have shape (nyg, nxg, nzg) (sigma fastest),
+plane fields (nyg, nxg).  The library must be built (``python build.py``); nothing here falls
+back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libow3d.so")
+_lib = None
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("alpha", ctypes.c_int), ("beta", ctypes.c_int), ("gamma", ctypes.c_int),
+        ("gridz", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("g", ctypes.c_double),
+        ("nonlinear", ctypes.c_int), ("solver", ctypes.c_int), ("precond", ctypes.c_int),
+        ("gmres_m", ctypes.c_int), ("reltol", ctypes.c_double), ("abstol", ctypes.c_double),
+        ("maxit", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
+        ("mg_maxgrids", ctypes.c_int), ("extrapolation", ctypes.c_int),
+        ("mg_precision", ctypes.c_int),
+    ]
+
+
+DEFAULTS = dict(alpha=3, beta=3, gamma=3, gridz=1, lx=1.0, ly=1.0, lz=1.0, g=9.81, nonlinear=1,
+                solver=1, precond=3, gmres_m=23, reltol=1e-8, abstol=0.0, maxit=1, nu1=1, nu2=1,
+                mg_maxgrids=20, extrapolation=1, mg_precision=0)
+
+EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_laplace_apply",
+           "ow3d_precond_apply", "ow3d_solve", "ow3d_set_surface", "ow3d_vertical_velocity",
+           "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
+           "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
+           "ow3d_time_matvec"]
+
+
+class OW3DError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libow3d.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OW3DError(f"{LIB_PATH} is missing: build it with oceanwave3d-fortran90_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        D = ctypes.POINTER(ctypes.c_double)
+        I = ctypes.c_int
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.ow3d_create.restype = I
+        L.ow3d_create.argtypes = [ctypes.POINTER(Config), D, D, D, D, D, ctypes.POINTER(P)]
+        L.ow3d_destroy.argtypes = [P]
+        L.ow3d_dims.argtypes = [P, IP, IP, IP, IP]
+        L.ow3d_last_error.restype = ctypes.c_char_p
+        L.ow3d_laplace_apply.argtypes = [P, D, D]
+        L.ow3d_precond_apply.argtypes = [P, D, D]
+        L.ow3d_solve.argtypes = [P, D, D, IP, D]
+        L.ow3d_set_surface.argtypes = [P, D, D, D, D, D, D, D, D]
+        L.ow3d_vertical_velocity.argtypes = [P, D, D]
+        L.ow3d_rhs_free_surface.argtypes = [P, D, D, D, D]
+        L.ow3d_add_relax_zone.argtypes = [P, I, I, I, I, I, D]
+        L.ow3d_set_state.argtypes = [P, D, D, D, D]
+        L.ow3d_get_state.argtypes = [P, D, D, D, D]
+        L.ow3d_rk4_steps.argtypes = [P, I, ctypes.c_double, ctypes.c_double]
+        L.ow3d_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), IP, IP]
+        L.ow3d_laplace_apply_device.argtypes = [P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.ow3d_stream.restype = ctypes.c_void_p
+        L.ow3d_stream.argtypes = [P]
+        L.ow3d_time_matvec.argtypes = [P, I, D]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise OW3DError(f"{what}: {lib().ow3d_last_error().decode()}")
+    return rc
+
+
+class Tank:
+    """One wave tank resident on the GPU."""
+
+    def __init__(self, cfg, h, hx=None, hy=None, hxx=None, hyy=None):
+        c = dict(DEFAULTS)
+        c.update(cfg)
+        self.cfg = c
+        self._c = Config(**c)
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (h, hx, hy, hxx, hyy)]
+        h_ = ctypes.c_void_p()
+        _check(lib().ow3d_create(ctypes.byref(self._c), *[_p(a) for a in arrs], ctypes.byref(h_)), "ow3d_create")
+        self.h = h_
+        d = [ctypes.c_int() for _ in range(4)]
+        lib().ow3d_dims(self.h, *[ctypes.byref(v) for v in d])
+        self.nxg, self.nyg, self.nzg, self.mg_levels = [v.value for v in d]
+        self.shape3 = (self.nyg, self.nxg, self.nzg)
+        self.shape2 = (self.nyg, self.nxg)
+        self.n = self.nyg * self.nxg * self.nzg
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ow3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def laplace_apply(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape3)
+        y = np.empty(self.shape3)
+        _check(lib().ow3d_laplace_apply(self.h, _p(phi), _p(y)), "ow3d_laplace_apply")
+        return y
+
+    def precond_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.shape3)
+        u = np.empty(self.shape3)
+        _check(lib().ow3d_precond_apply(self.h, _p(r), _p(u)), "ow3d_precond_apply")
+        return u
+
+    def solve(self, rhs, sol):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(self.shape3)
+        x = np.array(sol, dtype=np.float64, copy=True).reshape(self.shape3)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0.0)
+        st = _check(lib().ow3d_solve(self.h, _p(rhs), _p(x), ctypes.byref(it), ctypes.pointer(res)), "ow3d_solve")
+        return x, st, it.value, res.value
+
+    def set_surface(self, E, P):
+        E = np.array(E, dtype=np.float64, copy=True).reshape(self.shape2)
+        P = np.array(P, dtype=np.float64, copy=True).reshape(self.shape2)
+        out = {k: np.empty(self.shape2) for k in ("Ex", "Exx", "Ey", "Eyy", "Px", "Py")}
+        _check(lib().ow3d_set_surface(self.h, _p(E), _p(P), *[_p(out[k]) for k in ("Ex", "Exx", "Ey", "Eyy", "Px", "Py")]),
+               "ow3d_set_surface")
+        return E, P, out
+
+    def vertical_velocity(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape3)
+        W = np.empty(self.shape2)
+        _check(lib().ow3d_vertical_velocity(self.h, _p(phi), _p(W)), "ow3d_vertical_velocity")
+        return W
+
+    def rhs_free_surface(self, E, W):
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(self.shape2)
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.shape2)
+        rE = np.empty(self.shape2)
+        rP = np.empty(self.shape2)
+        _check(lib().ow3d_rhs_free_surface(self.h, _p(E), _p(W), _p(rE), _p(rP)), "ow3d_rhs_free_surface")
+        return rE, rP
+
+    def add_relax_zone(self, axis, i0, i1, j0, j1, gam):
+        gam = np.ascontiguousarray(gam, dtype=np.float64)
+        _check(lib().ow3d_add_relax_zone(self.h, axis, i0, i1, j0, j1, _p(gam)), "ow3d_add_relax_zone")
+
+    def set_state(self, E, P, W=None, phi=None):
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(self.shape2)
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(self.shape2)
+        W = None if W is None else np.ascontiguousarray(W, dtype=np.float64).reshape(self.shape2)
+        phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape3)
+        _check(lib().ow3d_set_state(self.h, _p(E), _p(P), _p(W), _p(phi)), "ow3d_set_state")
+
+    def get_state(self):
+        E = np.empty(self.shape2); P = np.empty(self.shape2); W = np.empty(self.shape2); phi = np.empty(self.shape3)
+        _check(lib().ow3d_get_state(self.h, _p(E), _p(P), _p(W), _p(phi)), "ow3d_get_state")
+        return E, P, W, phi
+
+    def rk4_steps(self, nsteps, t0, dt):
+        return _check(lib().ow3d_rk4_steps(self.h, int(nsteps), float(t0), float(dt)), "ow3d_rk4_steps")
+
+    def stats(self):
+        tot = ctypes.c_longlong(); sol = ctypes.c_longlong(); mn = ctypes.c_int(); mx = ctypes.c_int()
+        lib().ow3d_stats(self.h, ctypes.byref(tot), ctypes.byref(sol), ctypes.byref(mn), ctypes.byref(mx))
+        return tot.value, sol.value, mn.value, mx.value
+
+    def time_matvec(self, n_launches):
+        ms = ctypes.c_double(0.0)
+        _check(lib().ow3d_time_matvec(self.h, int(n_launches), ctypes.pointer(ms)), "ow3d_time_matvec")
+        return ms.value

@@ -1,0 +1,147 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the same inputs.
+
+Tolerances (FP64 path): matvec / plane operators / vertical velocity / FSBC rhs <= 1e-12 relative
+(L-inf, scaled by the L-inf of the output); FP64 multigrid V cycle <= 1e-10 relative to the oracle's
+zebra-line V cycle; converged Laplace solves <= 1e-9 relative; RK4 trajectories eta and Phi after N steps
+<= 1e-10 relative when both sides converge each solve to 1e-13.  FP32 multigrid: <= 1e-4 relative on M^-1 r.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _ow3d():
+    import ow3d
+    return ow3d
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def make_pair(oracle_mod, nx, ny, nz, shoal=False, order=6, L=10.0, mg_precision=0, **kw):
+    ow3d = _ow3d()
+    from ow3d import problems
+    A = order // 2
+    bath = (problems.gaussian_shoal if shoal else problems.flat_bottom)(nx, ny, L, L)
+    p = oracle_params(nx, ny, nz, alpha=A, beta=A, gamma=A, lx=L, ly=L, **kw)
+    T_o = oracle_mod.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+    cfg = {k: v for k, v in p.items() if k not in ("mg_mode", "cd_quirk")}
+    cfg["mg_precision"] = mg_precision
+    T_g = ow3d.Tank(cfg, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+    return T_o, T_g, bath
+
+
+def surface(nx, ny, L, seed=0, amp=0.02):
+    from ow3d import problems
+    x, y = problems.coords(nx, ny, L, L)
+    X, Y = np.meshgrid(x, y)
+    rng = np.random.default_rng(seed)
+    E = amp * np.sin(2 * np.pi * X / L + 0.3) * (np.cos(2 * np.pi * Y / L) if ny > 1 else 1.0)
+    E += 0.1 * amp * rng.standard_normal(E.shape)
+    P = 0.3 * np.cos(2 * np.pi * X / L) * (np.sin(2 * np.pi * Y / L + 0.2) if ny > 1 else 1.0)
+    return E, P
+
+
+CASES = [
+    (33, 33, 9, False, 6), (33, 33, 9, True, 6), (65, 1, 9, False, 6), (65, 1, 9, True, 6),
+    (29, 23, 9, True, 6), (33, 33, 12, True, 4), (33, 33, 6, True, 4), (41, 37, 12, True, 8),
+    (17, 17, 5, False, 4), (33, 1, 12, True, 8),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nz,shoal,order", CASES)
+def test_matvec_parity(oracle_mod, nx, ny, nz, shoal, order):
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, order=order, precond=0)
+    E, P = surface(nx, ny, 10.0)
+    Eo, Po, do = T_o.surface(E, P)
+    Eg, Pg, dg = T_g.set_surface(E, P)
+    assert rel(Eg, Eo) < 1e-14 and rel(Pg, Po) < 1e-14
+    for k in ("Ex", "Exx", "Px") + (("Ey", "Eyy", "Py") if ny > 1 else ()):
+        assert rel(dg[k], do[k]) < 1e-12, k
+    rng = np.random.default_rng(1)
+    phi = rng.standard_normal(T_o.shape3)
+    yo = T_o.matvec(phi)
+    yg = T_g.laplace_apply(phi)
+    assert rel(yg, yo) < 1e-12, (rel(yg, yo), np.unravel_index(np.argmax(np.abs(yg - yo)), yo.shape))
+
+
+@pytest.mark.parametrize("nx,ny,nz,shoal", [(33, 33, 9, True), (65, 1, 9, True), (17, 17, 5, False)])
+def test_vertical_velocity_and_rhs(oracle_mod, nx, ny, nz, shoal):
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, order=4 if nz < 7 else 6, precond=0)
+    E, P = surface(nx, ny, 10.0, seed=3)
+    Eo, Po, _ = T_o.surface(E, P)
+    Eg, Pg, _ = T_g.set_surface(E, P)
+    phi = np.random.default_rng(2).standard_normal(T_o.shape3)
+    Wo = T_o.vertical_velocity(phi)
+    Wg = T_g.vertical_velocity(phi)
+    assert rel(Wg, Wo) < 1e-13
+    ro = T_o.rhs(Eo, Po, Wo)
+    rg = T_g.rhs_free_surface(Eg, Wg)
+    assert rel(rg[0], ro[0]) < 1e-12 and rel(rg[1], ro[1]) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,nz,shoal", [(33, 33, 9, True), (65, 1, 9, True), (33, 17, 9, False)])
+def test_precond_fp64_parity(oracle_mod, nx, ny, nz, shoal):
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal)
+    r = np.random.default_rng(5).standard_normal(T_o.shape3)
+    uo = T_o.precond(r)
+    ug = T_g.precond_apply(r)
+    assert rel(ug, uo) < 1e-10
+
+
+def test_precond_fp32(oracle_mod):
+    T_o, T_g, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=1)
+    r = np.random.default_rng(6).standard_normal(T_o.shape3)
+    uo = T_o.precond(r)
+    ug = T_g.precond_apply(r)
+    assert rel(ug, uo) < 1e-4
+
+
+@pytest.mark.parametrize("solver", [1, 0])
+@pytest.mark.parametrize("nx,ny,nz,shoal", [(33, 33, 9, True), (65, 1, 9, False)])
+def test_solve_parity(oracle_mod, nx, ny, nz, shoal, solver):
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, solver=solver, reltol=1e-13, gmres_m=31 if solver else 31)
+    E, P = surface(nx, ny, 10.0, seed=4)
+    T_o.surface(E, P)
+    T_g.set_surface(E, P)
+    b = np.zeros(T_o.shape3)
+    if ny > 1:
+        b[1:-1, 1:-1, -1] = P[1:-1, 1:-1]
+    else:
+        b[:, 1:-1, -1] = P[:, 1:-1]
+    x0 = np.zeros(T_o.shape3)
+    xo, sto, ito, _ = T_o.solve(b, x0)
+    xg, stg, itg, _ = T_g.solve(b, x0)
+    ref = T_o.matvec(xg)
+    assert rel(ref, b) < 1e-8
+    assert rel(xg, xo) < 1e-8, (rel(xg, xo), ito, itg)
+    assert abs(ito - itg) <= 2
+
+
+@pytest.mark.parametrize("nx,ny,nz,shoal,sponge", [(33, 33, 9, True, True), (65, 1, 9, False, False)])
+def test_rk4_parity(oracle_mod, nx, ny, nz, shoal, sponge):
+    from ow3d import problems
+    L = 10.0
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, reltol=1e-13, L=L)
+    E, P = surface(nx, ny, L, seed=7, amp=0.01)
+    P = 0.01 * P
+    if sponge:
+        for (ax, i0, i1, j0, j1, gam) in problems.sponge_zones(nx, ny, L, L, 2.0):
+            T_o.add_relax_zone(ax, i0, i1, j0, j1, gam)
+            T_g.add_relax_zone(ax, i0, i1, j0, j1, gam)
+    T_o.set_state(E, P)
+    T_g.set_state(E, P)
+    dt = 0.05
+    nsteps = 5
+    for s in range(nsteps):
+        T_o.rk4_step(s * dt, dt)
+    T_g.rk4_steps(nsteps, 0.0, dt)
+    Eo, Po, Wo, phio = T_o.get_state()
+    Eg, Pg, Wg, phig = T_g.get_state()
+    assert rel(Eg, Eo) < 1e-10, rel(Eg, Eo)
+    assert rel(Pg, Po) < 1e-10, rel(Pg, Po)
+    assert rel(Wg, Wo) < 1e-9
