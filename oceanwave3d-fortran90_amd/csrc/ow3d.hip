@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -33,7 +34,7 @@ extern "C" const char *ow3d_last_error(void) { return g_err.c_str(); }
 namespace {
 
 constexpr int MAXK = 32;     // Krylov basis limit (GMRESmaxiterations <= MAXK - 1)
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 2048;
 
 template <typename T>
 struct DevBuf {
@@ -93,7 +94,10 @@ struct ow3d_tank {
     int it_min = -1, it_max = 0;
     long tstep = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool use_graph = true;
+    hipGraphExec_t pg_exec = nullptr;   // captured preconditioner (V cycles)
     ~ow3d_tank() {
+        if (pg_exec) (void)hipGraphExecDestroy(pg_exec);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (s) (void)hipStreamDestroy(s);
@@ -107,8 +111,8 @@ namespace {
 
 template <int NZ, int A>
 int launch_mv3(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
-    constexpr int R = A + 1, NZP = NZ | 1;
-    constexpr size_t lds16 = (size_t)(16 + 2 * R) * (16 + 2 * R) * NZP * 8;
+    constexpr int R = A + 1, RA = 2 * A + 1;
+    constexpr size_t lds16 = ((size_t)NZ * (((16 + 2 * R) * (16 + 2 * R)) | 1) + 36 * RA) * 8;
     constexpr int TY = lds16 <= 65536 ? 16 : 8;
     dim3 grid((t->g.nxg + 15) / 16, (t->g.nyg + TY - 1) / TY);
     hipLaunchKernelGGL((k_matvec3<NZ, A, TY>), grid, dim3(16 * TY), 0, s, t->mv, phi, y);
@@ -145,13 +149,12 @@ int launch_matvec(const ow3d_tank *t, const double *phi, double *y, hipStream_t 
         case 12: return dispatch_mv_a<12>(t, phi, y, s);
         case 13: return dispatch_mv_a<13>(t, phi, y, s);
         case 14: return dispatch_mv_a<14>(t, phi, y, s);
-        case 17: return dispatch_mv_a<17>(t, phi, y, s);
     }
     return -1;
 }
 
 bool matvec_supported(int nzg, int A) {
-    static const int nz_ok[] = {5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 17};
+    static const int nz_ok[] = {5, 6, 7, 8, 9, 10, 11, 12, 13, 14};
     bool ok = false;
     for (int v : nz_ok) ok |= (v == nzg);
     return ok && A >= 1 && A <= 4 && 2 * A + 1 <= nzg;
@@ -160,12 +163,24 @@ bool matvec_supported(int nzg, int A) {
 template <typename T, int NZ>
 void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, hipStream_t s) {
     if (what == 0) {
-        hipLaunchKernelGGL((k_mg_residual<T, NZ>), dim3(grid_blocks(L.nxy)), dim3(256), 0, s, L, x, b, r);
+        if (L.dim3) {
+            dim3 grid((L.nxg + 15) / 16, (L.nyg + 15) / 16);
+            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
+        } else {
+            dim3 grid((L.nxg + 255) / 256, 1);
+            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
+        }
     } else {
         const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-        const int half = (nxi + 1) / 2;
-        dim3 grid((half + 63) / 64, nyi);
-        hipLaunchKernelGGL((k_mg_zebra<T, NZ>), grid, dim3(64), 0, s, L, colour, x, b);
+        constexpr size_t lds = (size_t)(34 * 10 + 32 * 8) * (NZ | 1) * sizeof(T);
+        if constexpr (lds <= 65536) {
+            dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
+            hipLaunchKernelGGL((k_mg_zebra_t<T, NZ>), grid, dim3(128), 0, s, L, colour, x, b);
+        } else {
+            const int half = (nxi + 1) / 2;
+            dim3 grid((half + 63) / 64, nyi);
+            hipLaunchKernelGGL((k_mg_zebra<T, NZ>), grid, dim3(64), 0, s, L, colour, x, b);
+        }
     }
 }
 
@@ -339,14 +354,19 @@ RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const double *yf
 }
 
 template <typename T>
-int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, hipStream_t s) {
+void ghost_pass(MgLevel<T> &L, hipStream_t s) {
     const Grid &g = L.g;
-    const int nb = g.ny > 1 ? 2 * g.nxg + 2 * (g.nyg - 2) : 2;
-    hipLaunchKernelGGL((k_mg_ghost<T>), dim3((nb + 63) / 64), dim3(64), 0, s, L.args, L.x.p, (const T *)L.b.p);
+    const int nb = (g.ny > 1 ? 2 * g.nxg + 2 * (g.nyg - 2) : 2) * g.nzg;
+    hipLaunchKernelGGL((k_mg_ghost_p<T>), dim3((nb + 255) / 256), dim3(256), 0, s, L.args, L.x.p, (const T *)L.b.p);
+}
+
+template <typename T>
+int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, hipStream_t s) {
+    ghost_pass(L, s);
     for (int it = 0; it < nu; it++) {
         if (mg_launch<T>(L.args, 1, 0, L.x.p, L.b.p, nullptr, s)) return -1;
         if (mg_launch<T>(L.args, 1, 1, L.x.p, L.b.p, nullptr, s)) return -1;
-        hipLaunchKernelGGL((k_mg_ghost<T>), dim3((nb + 63) / 64), dim3(64), 0, s, L.args, L.x.p, (const T *)L.b.p);
+        ghost_pass(L, s);
     }
     (void)t;
     return 0;
@@ -371,23 +391,19 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, hi
     return 0;
 }
 
-// u = M^-1 r (device vectors, FP64 in / out)
-int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
+// The preconditioner body: input r (FP64) -> finest-level right-hand side, V cycles from x = 0.
+// Only kernel launches (no memset / memcpy nodes), so the sequence can be captured in a hipGraph.
+int precond_body(ow3d_tank *t, const double *r, hipStream_t s) {
     const size_t n = t->g.n;
-    if (t->cfg.precond != 3) {
-        HIPCHK(hipMemcpyAsync(u, r, n * 8, hipMemcpyDeviceToDevice, s));
-        return 0;
-    }
     const int top = t->nlev - 1;
     if (t->cfg.mg_precision == 0) {
         MgLevel<double> &F = *t->mg64[top];
-        HIPCHK(hipMemcpyAsync(F.b.p, r, n * 8, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p);
         hipLaunchKernelGGL((k_fill<double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, 0.0, F.x.p);
         for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
             if (t->nlev == 1 && it > 0) break;
             if (vcycle<double>(t, t->mg64, top, s)) return -1;
         }
-        HIPCHK(hipMemcpyAsync(u, F.x.p, n * 8, hipMemcpyDeviceToDevice, s));
     } else {
         MgLevel<float> &F = *t->mg32[top];
         hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p);
@@ -396,8 +412,38 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
             if (t->nlev == 1 && it > 0) break;
             if (vcycle<float>(t, t->mg32, top, s)) return -1;
         }
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)F.x.p, u);
     }
+    return 0;
+}
+
+// u = M^-1 r (device vectors, FP64 in / out).  When r is the handle's residual buffer the V cycle is
+// replayed from a hipGraph captured on first use (one graph per handle; fixed buffers).
+int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
+    const size_t n = t->g.n;
+    if (t->cfg.precond != 3) {
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, u);
+        return 0;
+    }
+    if (t->use_graph && r == t->w.p && s == t->s) {
+        if (!t->pg_exec) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = precond_body(t, r, s);
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return -1; }
+            HIPCHK(e);
+            HIPCHK(hipGraphInstantiate(&t->pg_exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+        }
+        HIPCHK(hipGraphLaunch(t->pg_exec, s));
+    } else {
+        if (precond_body(t, r, s)) return -1;
+    }
+    const int top = t->nlev - 1;
+    if (t->cfg.mg_precision == 0)
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg64[top]->x.p, u);
+    else
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg32[top]->x.p, u);
     return 0;
 }
 
@@ -410,8 +456,8 @@ int red_blocks(size_t n) { return (int)std::min<size_t>(RED_BLOCKS, std::max<siz
 int mdot(ow3d_tank *t, int k, const double *V, const double *w, double *vals, hipStream_t s) {
     const size_t n = t->g.n;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_mdot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, V, w, t->part.p);
-    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(1), dim3(64), 0, s, nb, k, (const double *)t->part.p, t->hv.p);
+    hipLaunchKernelGGL((k_mdot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, V, w, t->part.p, 0);
+    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)t->part.p, t->hv.p);
     HIPCHK(hipMemcpyAsync(vals, t->hv.p, k * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return 0;
@@ -436,7 +482,7 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
     int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
     double *V = t->V.p, *w = t->w.p;
     const int nbl = grid_blocks(n);
-    std::vector<double> H((size_t)(m + 1) * (m + 1), 0.0), cs(m + 1), sn(m + 1), gv(m + 2, 0.0), h(MAXK), h2(MAXK);
+    std::vector<double> H((size_t)(m + 1) * (m + 1), 0.0), cs(m + 1), sn(m + 1), gv(m + 2, 0.0), h(MAXK + 1), h2(MAXK + 1);
     int matvecs = 0, k = 0;
     if (launch_matvec(t, x, w, s)) { g_err = "matvec dispatch failed"; return -1; }
     matvecs++;
@@ -458,19 +504,31 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
         if (launch_matvec(t, vk, w, s)) { g_err = "matvec dispatch failed"; return -1; }
         matvecs++;
         if (precond(t, w, vn, s)) return -1;
-        // pass 1: h = V^T vn
-        if (mdot(t, k, V, vn, h.data(), s)) return -1;          // leaves h in t->hv
-        // pass 2: vn -= V h ; h2 = V^T vn ; ||vn||^2
-        hipLaunchKernelGGL((k_gs_update_dot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)t->hv.p, vn, t->part.p);
-        hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(1), dim3(64), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
-        HIPCHK(hipMemcpyAsync(h2.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
+        // classical Gram-Schmidt with the DGKS re-orthogonalisation test (second pass only when the
+        // projection removed more than half of the squared norm)
+        hipLaunchKernelGGL((k_mdot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)vn, t->part.p, 1);
+        hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
+        HIPCHK(hipMemcpyAsync(h.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        double nn = h2[k];
-        for (int i = 0; i < k; i++) nn -= h2[i] * h2[i];
-        double hn = std::sqrt(std::max(nn, 0.0));
-        for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i] + h2[i];
+        const double nrm2 = h[k];
+        double nn = nrm2;
+        for (int i = 0; i < k; i++) nn -= h[i] * h[i];
+        double hn;
+        if (nn > 0.5 * nrm2) {
+            hn = std::sqrt(nn);
+            for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i];
+        } else {
+            hipLaunchKernelGGL((k_gs_update_dot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)t->hv.p, vn, t->part.p);
+            hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
+            HIPCHK(hipMemcpyAsync(h2.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            double n2 = h2[k];
+            for (int i = 0; i < k; i++) n2 -= h2[i] * h2[i];
+            hn = std::sqrt(std::max(n2, 0.0));
+            for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i] + h2[i];
+        }
         H[(size_t)k * m + (k - 1)] = hn;
-        // vn = (vn - V h2) / hn
+        // vn = (vn - V h) / hn  with the h of the last pass (in t->hv)
         hipLaunchKernelGGL((k_gs_update<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)t->hv.p, -1.0,
                            hn > 0.0 ? 1.0 / hn : 1.0, vn);
         for (int i = 0; i < k - 1; i++) {
@@ -589,6 +647,10 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (c.precond != 0 && c.precond != 3) { g_err = "precond must be 0 or 3"; return -1; }
     auto t = std::make_unique<ow3d_tank>();
     t->cfg = c;
+    {
+        const char *ng = std::getenv("OW3D_NO_GRAPH");
+        t->use_graph = !(ng && ng[0] == '1');
+    }
     Grid &g = t->g;
     g.nx = c.nx; g.ny = c.ny; g.nz = c.nz; g.gx = 1; g.gy = c.ny > 1 ? 1 : 0;
     g.a = c.alpha; g.b = c.ny > 1 ? c.beta : 0; g.c = c.gamma;
@@ -711,8 +773,8 @@ extern "C" int ow3d_time_matvec(ow3d_tank *t, int n_launches, double *avg_ms) {
 extern "C" int ow3d_precond_apply(ow3d_tank *t, const double *r, double *u) {
     if (!t || !r || !u) { g_err = "null argument"; return -1; }
     const size_t n = t->g.n;
-    HIPCHK(hipMemcpyAsync(t->io_a.p, r, n * 8, hipMemcpyHostToDevice, t->s));
-    if (precond(t, t->io_a.p, t->io_b.p, t->s)) return -1;
+    HIPCHK(hipMemcpyAsync(t->w.p, r, n * 8, hipMemcpyHostToDevice, t->s));
+    if (precond(t, t->w.p, t->io_b.p, t->s)) return -1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(u, t->io_b.p, n * 8, hipMemcpyDeviceToHost, t->s));
     HIPCHK(hipStreamSynchronize(t->s));

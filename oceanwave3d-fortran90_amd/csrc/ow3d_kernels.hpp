@@ -53,27 +53,40 @@ struct MvArgs {
     const double *bath;          // 3 planes: h_x, h_y, h_xx+h_yy (nullptr: flat bottom)
 };
 
-// 3D: block = 16 x TY columns, one thread per sigma column.
+// 3D: block = 16 x TY columns, one thread per sigma column.  The tile is kept level-major in LDS
+// (plane stride PS odd, so lanes of a wave touch consecutive words on every access); the stencil
+// rows of the tile's columns and rows are staged in LDS as well.
 template <int NZ, int A, int TY>
 __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
-    constexpr int TX = 16, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1;
-    __shared__ double sP[WX * WY * NZP];
+    constexpr int TX = 16, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R, PS = (WX * WY) | 1;
+    __shared__ double sP[NZ * PS];
+    __shared__ double sSX[(TX + 2) * 2 * RA];
+    __shared__ double sSY[(TY + 2) * 2 * RA];
     const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
     const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
-    {
-        const int rowlen = wx * NZ, tot = rowlen * wy;
-        for (int f = threadIdx.x; f < tot; f += blockDim.x) {
-            const int jj = f / rowlen, r = f - jj * rowlen;
-            const int ii = r / NZ, k = r - ii * NZ;
-            sP[(jj * WX + ii) * NZP + k] = phi[((size_t)(loy + jj) * nxg + lox) * NZ + r];
+    for (int jj = 0; jj < wy; jj++) {
+        const double *src = phi + ((size_t)(loy + jj) * nxg + lox) * NZ;
+        for (int e = threadIdx.x; e < wx * NZ; e += blockDim.x) {
+            const int ii = e / NZ, k = e - ii * NZ;
+            sP[k * PS + jj * WX + ii] = src[e];
         }
     }
+    for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += blockDim.x) {
+        const int r = e / (2 * RA), ii = i0 - 1 + r;
+        if (ii >= 0 && ii < nxg) sSX[e] = g.sx[(size_t)ii * 2 * RA + (e - r * 2 * RA)];
+    }
+    for (int e = threadIdx.x; e < (TY + 2) * 2 * RA; e += blockDim.x) {
+        const int r = e / (2 * RA), jj = j0 - 1 + r;
+        if (jj >= 0 && jj < nyg) sSY[e] = g.sy[(size_t)jj * 2 * RA + (e - r * 2 * RA)];
+    }
     __syncthreads();
-    auto P = [&](int k, int ii, int jj) -> double { return sP[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
+    auto P = [&](int k, int ii, int jj) -> double { return sP[k * PS + (jj - loy) * WX + (ii - lox)]; };
+    auto SX = [&](int ii) -> const double * { return sSX + (ii - (i0 - 1)) * 2 * RA; };
+    auto SY = [&](int jj) -> const double * { return sSY + (jj - (j0 - 1)) * 2 * RA; };
     const int i = i0 + tx, j = j0 + ty;
     const bool active = (i < nxg) && (j < nyg);
     double o[NZ];
@@ -84,7 +97,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
             for (int k = 0; k < NZ; k++) o[k] = P(k, i, j);
         } else if (wxl) {
             const int ib = (i == 0) ? 1 : nxg - 2, ws = cclamp(ib - A, 0, nxg - RA);
-            const double *S = g.sx + (size_t)(ib * 2) * RA;
+            const double *S = SX(ib);
             o[0] = P(0, i, j);
 #pragma unroll
             for (int k = 1; k < NZ; k++) {
@@ -95,7 +108,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
             }
         } else if (wyl) {
             const int jb = (j == 0) ? 1 : nyg - 2, ws = cclamp(jb - A, 0, nyg - RA);
-            const double *S = g.sy + (size_t)(jb * 2) * RA;
+            const double *S = SY(jb);
             o[0] = P(0, i, j);
 #pragma unroll
             for (int k = 1; k < NZ; k++) {
@@ -105,34 +118,26 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
                 o[k] = s;
             }
         } else {
-            double c[NZ], dx[NZ], dy[NZ], axx[NZ], ayy[NZ];
-#pragma unroll
-            for (int k = 0; k < NZ; k++) c[k] = P(k, i, j);
+            double dx[NZ], dy[NZ];
             const int wsx = cclamp(i - A, 0, nxg - RA), wsy = cclamp(j - A, 0, nyg - RA);
             {
-                const double *S1 = g.sx + (size_t)(i * 2) * RA, *S2 = S1 + RA;
-                double a1[RA], a2[RA];
-#pragma unroll
-                for (int m = 0; m < RA; m++) { a1[m] = S1[m]; a2[m] = S2[m]; }
+                const double *S1 = SX(i), *S2 = S1 + RA;
 #pragma unroll
                 for (int k = 0; k < NZ; k++) {
                     double d1 = 0.0, d2 = 0.0;
 #pragma unroll
-                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += a1[m] * v; d2 += a2[m] * v; }
-                    dx[k] = d1; axx[k] = d2;
+                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += S1[m] * v; d2 += S2[m] * v; }
+                    dx[k] = d1; o[k] = d2;
                 }
             }
             {
-                const double *S1 = g.sy + (size_t)(j * 2) * RA, *S2 = S1 + RA;
-                double a1[RA], a2[RA];
-#pragma unroll
-                for (int m = 0; m < RA; m++) { a1[m] = S1[m]; a2[m] = S2[m]; }
+                const double *S1 = SY(j), *S2 = S1 + RA;
 #pragma unroll
                 for (int k = 0; k < NZ; k++) {
                     double d1 = 0.0, d2 = 0.0;
 #pragma unroll
-                    for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); d1 += a1[m] * v; d2 += a2[m] * v; }
-                    dy[k] = d1; ayy[k] = d2;
+                    for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); d1 += S1[m] * v; d2 += S2[m] * v; }
+                    dy[k] = d1; o[k] = o[k] + d2;
                 }
             }
             // corner-corrected first pass on the sigma ghost level (ConstructTableCrossDerivatives.f90:173-186)
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
                 if (wsx == 0) sh = 1;
                 if (wsx + RA - 1 == nxg - 1) sh = -1;
                 if (sh != 0) {
-                    const double *S = g.sx + (size_t)((i - sh) * 2) * RA;
+                    const double *S = SX(i - sh);
                     double s = 0.0;
 #pragma unroll
                     for (int m = 0; m < RA; m++) s += S[m] * P(0, wsx + sh + m, j);
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
                 if (wsy == 0) sh = 1;
                 if (wsy + RA - 1 == nyg - 1) sh = -1;
                 if (sh != 0) {
-                    const double *S = g.sy + (size_t)((j - sh) * 2) * RA;
+                    const double *S = SY(j - sh);
                     double s = 0.0;
 #pragma unroll
                     for (int m = 0; m < RA; m++) s += S[m] * P(0, i, wsy + sh + m);
@@ -163,51 +168,52 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__r
             const double d = g.cf[p], Ex = g.cf[nxy + p], Ey = g.cf[2 * nxy + p], Es = g.cf[3 * nxy + p];
             double hx = 0.0, hy = 0.0, hs = 0.0;
             if (g.bath) { hx = g.bath[p]; hy = g.bath[nxy + p]; hs = g.bath[2 * nxy + p]; }
-            o[NZ - 1] = c[NZ - 1];
-#pragma unroll
-            for (int k = 1; k < NZ - 1; k++) {
-                constexpr int dummy = 0; (void)dummy;
-                const int wz = cclamp(k - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (size_t)(k * 2) * RA, *Z2 = Z1 + RA;
-                double dz = 0.0, dzz = 0.0, dxz = 0.0, dyz = 0.0;
-#pragma unroll
-                for (int m = 0; m < RA; m++) {
-                    const double z1 = Z1[m];
-                    dz += z1 * c[wz + m];
-                    dzz += Z2[m] * c[wz + m];
-                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
-                    dyz += z1 * ((wz + m == 0) ? dyc0 : dy[wz + m]);
-                }
-                double s2, s3, s4, s5;
-                sigma_coef3<double>(g.zs[k], d, hx, hy, hs, Ex, Ey, Es, s2, s3, s4, s5);
-                o[k] = axx[k] + ayy[k] + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz;
-            }
+            double obot;
             {
                 const int wz = cclamp(1 - A, 0, NZ - RA);
                 const double *Z1 = g.sz + (size_t)(1 * 2) * RA;
                 double dz = 0.0;
 #pragma unroll
-                for (int m = 0; m < RA; m++) dz += Z1[m] * c[wz + m];
+                for (int m = 0; m < RA; m++) dz += Z1[m] * P(wz + m, i, j);
                 double s2, s3, s4, s5;
                 sigma_coef3<double>(g.zs[1], d, hx, hy, hs, Ex, Ey, Es, s2, s3, s4, s5);
-                o[0] = (s5 + hx * s2 + hy * s4) * dz + hx * dx[1] + hy * dy[1];
+                obot = (s5 + hx * s2 + hy * s4) * dz + hx * dx[1] + hy * dy[1];
             }
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const int wz = cclamp(k - A, 0, NZ - RA);
+                const double *Z1 = g.sz + (size_t)(k * 2) * RA, *Z2 = Z1 + RA;
+                double dz = 0.0, dzz = 0.0, dxz = 0.0, dyz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) {
+                    const double z1 = Z1[m], cv = P(wz + m, i, j);
+                    dz += z1 * cv;
+                    dzz += Z2[m] * cv;
+                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
+                    dyz += z1 * ((wz + m == 0) ? dyc0 : dy[wz + m]);
+                }
+                double s2, s3, s4, s5;
+                sigma_coef3<double>(g.zs[k], d, hx, hy, hs, Ex, Ey, Es, s2, s3, s4, s5);
+                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz;
+            }
+            o[0] = obot;
+            o[NZ - 1] = P(NZ - 1, i, j);
         }
     }
     __syncthreads();
     if (active) {
-        double *dst = sP + ((j - loy) * WX + (i - lox)) * NZP;
 #pragma unroll
-        for (int k = 0; k < NZ; k++) dst[k] = o[k];
+        for (int k = 0; k < NZ; k++) sP[k * PS + (j - loy) * WX + (i - lox)] = o[k];
     }
     __syncthreads();
     {
         const int ie = cmin(i0 + TX, nxg), je = cmin(j0 + TY, nyg);
-        const int rowlen = (ie - i0) * NZ, tot = rowlen * (je - j0);
-        for (int f = threadIdx.x; f < tot; f += blockDim.x) {
-            const int jj = f / rowlen, r = f - jj * rowlen;
-            const int ii = r / NZ, k = r - ii * NZ;
-            out[((size_t)(j0 + jj) * nxg + i0) * NZ + r] = sP[((j0 + jj - loy) * WX + (i0 + ii - lox)) * NZP + k];
+        for (int jj = j0; jj < je; jj++) {
+            double *dst = out + ((size_t)jj * nxg + i0) * NZ;
+            for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+                const int ii = e / NZ, k = e - ii * NZ;
+                dst[e] = sP[k * PS + (jj - loy) * WX + (i0 + ii - lox)];
+            }
         }
     }
 }
@@ -523,18 +529,22 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[MAXK], int kk, 
     }
 }
 
-// part[b][i] = sum over block b's range of V_i . w, i < k  (V_i = V + i*n)
+// part[b][i] = sum over block b's range of V_i . w, i < k  (V_i = V + i*n); with_norm adds w . w at index k.
 template <int MAXK>
-__global__ __launch_bounds__(256) void k_mdot(size_t n, int k, const double *__restrict__ V, const double *__restrict__ w, double *part) {
+__global__ __launch_bounds__(256) void k_mdot(size_t n, int k, const double *__restrict__ V, const double *__restrict__ w, double *part,
+                                               int with_norm) {
     double acc[MAXK];
 #pragma unroll
     for (int i = 0; i < MAXK; i++) acc[i] = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         const double wq = w[q];
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) if (i < k) acc[i] += V[(size_t)i * n + q] * wq;
+        for (int i = 0; i < MAXK; i++) {
+            if (i < k) acc[i] += V[(size_t)i * n + q] * wq;
+            else if (i == k && with_norm) acc[i] += wq * wq;
+        }
     }
-    block_reduce_store<MAXK>(acc, k, part);
+    block_reduce_store<MAXK>(acc, k + (with_norm ? 1 : 0), part);
 }
 
 // v <- v - sum_i h_i V_i; part gets the dots of the new v with V_i (i < k) and its squared norm (index k).
@@ -574,13 +584,19 @@ __global__ __launch_bounds__(256) void k_gs_update(size_t n, int k, const double
     }
 }
 
+// out[i] = sum_b part[b][i]; one block per entry, fixed summation order (deterministic).
 template <int MAXK>
-__global__ void k_reduce_parts(int nblk, int k, const double *part, double *out) {
-    const int i = threadIdx.x;
+__global__ __launch_bounds__(256) void k_reduce_parts(int nblk, int k, const double *part, double *out) {
+    const int i = blockIdx.x;
     if (i >= k) return;
     double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += part[(size_t)b * MAXK + i];
-    out[i] = s;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += part[(size_t)b * MAXK + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = ((w[0] + w[1]) + w[2]) + w[3];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -773,6 +789,290 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
     for (int k = NZ - 2; k >= 0; k--) xn[k] = (rh[k] - up[k] * xn[k + 1]) / di[k];
 #pragma unroll
     for (int k = 0; k < NZ; k++) xc[k] = xn[k];
+}
+
+// LDS-tiled zebra z-line relaxation: a block stages a 32 x 8 tile of columns (plus a one-column
+// halo) and the tile's right-hand sides, 128 threads each solve the sweep-colour column of one
+// (even, odd) pair, and the whole tile is written back coalesced (columns of the other colour
+// are rewritten unchanged; nothing else modifies them during the sweep).
+template <typename T, int NZ>
+__global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    constexpr int TX = 32, TY = 8, WX = TX + 2, WY = TY + 2, NZP = NZ | 1;
+    __shared__ T sX[WX * WY * NZP];
+    __shared__ T sB[TX * TY * NZP];
+    const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
+    const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
+    const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
+    const int lox = i0 - 1, loy = d3 ? j0 - 1 : 0;
+    const int wcols = cmin(WX, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
+    for (int r = 0; r < wrows; r++) {
+        const T *src = x + ((size_t)(loy + r) * nxg + lox) * NZ;
+        for (int e = threadIdx.x; e < wcols * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            sX[(r * WX + c) * NZP + k] = src[e];
+        }
+    }
+    for (int r = 0; r < je - j0; r++) {
+        const T *src = b + ((size_t)(j0 + r) * nxg + i0) * NZ;
+        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            sB[(r * TX + c) * NZP + k] = src[e];
+        }
+    }
+    __syncthreads();
+    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
+    const int ty = threadIdx.x / 16, pr = threadIdx.x % 16;
+    const int j = j0 + ty;
+    int i = i0 + 2 * pr;
+    if (((i + j) & 1) != colour) i++;
+    if (j < je && i < ie) {
+        const int p = i + nxg * j;
+        const T *bc = sB + ((j - j0) * TX + (i - i0)) * NZP;
+        const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
+        const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
+        const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
+        int shx = 0; if (wsx == 0) shx = 1; if (wsx + 2 == nxg - 1) shx = -1;
+        int shy = 0;
+        if (d3) { if (wsy == 0) shy = 1; if (wsy + 2 == nyg - 1) shy = -1; }
+        const T *SXc = L.sx + ((i - shx) * 2) * 3;
+        const T *SYc = L.sy + ((j - shy) * 2) * 3;
+        T lo[NZ], di[NZ], up[NZ], rh[NZ];
+        T e2 = T(0);
+#pragma unroll
+        for (int k = 0; k < NZ; k++) { lo[k] = T(0); di[k] = T(0); up[k] = T(0); }
+        {
+            const T *Sz = L.sz + (1 * 2) * 3;
+            T s2, s3, s4, s5;
+            low_coefs(L, p, 1, s2, s3, s4, s5);
+            const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
+            const T cz = s5 + hx * s2 + hy * s4;
+            T a0 = cz * Sz[0], a1 = cz * Sz[1], a2 = cz * Sz[2], off = T(0);
+            for (int m = 0; m < 3; m++) {
+                if (wsx + m == i) a1 += hx * SX1[m]; else off += hx * SX1[m] * X(1, wsx + m, j);
+            }
+            if (d3)
+                for (int m = 0; m < 3; m++) {
+                    if (wsy + m == j) a1 += hy * SY1[m]; else off += hy * SY1[m] * X(1, i, wsy + m);
+                }
+            di[0] = a0; up[0] = a1; e2 = a2;
+            rh[0] = bc[0] - off;
+        }
+#pragma unroll
+        for (int k = 1; k < NZ - 1; k++) {
+            const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
+            T s2, s3, s4, s5;
+            low_coefs(L, p, k, s2, s3, s4, s5);
+            const T czz = s2 * s2 + s4 * s4 + s5 * s5;
+            T a[3];
+            for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
+            T off = T(0);
+            for (int m = 0; m < 3; m++) {
+                if (wsx + m == i) a[1] += SX2[m]; else off += SX2[m] * X(k, wsx + m, j);
+            }
+            for (int mz = 0; mz < 3; mz++) {
+                const int kk = k - 1 + mz;
+                const int sh = (kk == 0) ? shx : 0;
+                const T *S1 = (kk == 0) ? SXc : SX1;
+                for (int m = 0; m < 3; m++) {
+                    const T w = T(2) * s2 * Z1[mz] * S1[m];
+                    const int ii = wsx + sh + m;
+                    if (ii == i) a[mz] += w;
+                    else if (ii - i != 2 && i - ii != 2) off += w * X(kk, ii, j);
+                }
+            }
+            if (d3) {
+                for (int m = 0; m < 3; m++) {
+                    if (wsy + m == j) a[1] += SY2[m]; else off += SY2[m] * X(k, i, wsy + m);
+                }
+                for (int mz = 0; mz < 3; mz++) {
+                    const int kk = k - 1 + mz;
+                    const int sh = (kk == 0) ? shy : 0;
+                    const T *S1 = (kk == 0) ? SYc : SY1;
+                    for (int m = 0; m < 3; m++) {
+                        const T w = T(2) * s4 * Z1[mz] * S1[m];
+                        const int jj = wsy + sh + m;
+                        if (jj == j) a[mz] += w;
+                        else if (jj - j != 2 && j - jj != 2) off += w * X(kk, i, jj);
+                    }
+                }
+            }
+            lo[k] = a[0]; di[k] = a[1]; up[k] = a[2];
+            rh[k] = bc[k] - off;
+        }
+        di[NZ - 1] = T(1); rh[NZ - 1] = bc[NZ - 1];
+        if (e2 != T(0)) {
+            const T f = e2 / up[1];
+            di[0] -= f * lo[1]; up[0] -= f * di[1]; rh[0] -= f * rh[1];
+        }
+#pragma unroll
+        for (int k = 1; k < NZ; k++) {
+            const T f = lo[k] / di[k - 1];
+            di[k] -= f * up[k - 1];
+            rh[k] -= f * rh[k - 1];
+        }
+        T *own = sX + ((j - loy) * WX + (i - lox)) * NZP;
+        T xn = rh[NZ - 1] / di[NZ - 1];
+        own[NZ - 1] = xn;
+#pragma unroll
+        for (int k = NZ - 2; k >= 0; k--) { xn = (rh[k] - up[k] * xn) / di[k]; own[k] = xn; }
+    }
+    __syncthreads();
+    for (int r = 0; r < je - j0; r++) {
+        T *dst = x + ((size_t)(j0 + r) * nxg + i0) * NZ;
+        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
+        }
+    }
+}
+
+// Wall ghost points, one thread per (ghost column, level): Neumann rows relaxed pointwise, identity
+// rows set to the right-hand side (the role of GaussSeidelSpecialPoints.f90).
+template <typename T>
+__global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxg = L.nxg, nyg = L.nyg, nzg = L.nzg;
+    const int col = t / nzg, k = t - col * nzg;
+    int i, j;
+    if (L.dim3) {
+        const int nb = 2 * nxg + 2 * (nyg - 2);
+        if (col >= nb) return;
+        if (col < nxg) { i = col; j = 0; }
+        else if (col < 2 * nxg) { i = col - nxg; j = nyg - 1; }
+        else if (col < 2 * nxg + nyg - 2) { i = 0; j = 1 + col - 2 * nxg; }
+        else { i = nxg - 1; j = 1 + col - 2 * nxg - (nyg - 2); }
+    } else {
+        if (col >= 2) return;
+        i = col == 0 ? 0 : nxg - 1; j = 0;
+    }
+    const size_t q = ((size_t)j * nxg + i) * nzg + k;
+    const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
+    if ((wx && wy) || k == 0) { x[q] = b[q]; return; }
+    T s = T(0), dg = T(1);
+    if (wx) {
+        const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
+        const T *S = L.sx + (ib * 2) * 3;
+        for (int m = 0; m < 3; m++) {
+            if (ws + m == i) dg = S[m];
+            else s += S[m] * x[((size_t)j * nxg + ws + m) * nzg + k];
+        }
+    } else {
+        const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
+        const T *S = L.sy + (jb * 2) * 3;
+        for (int m = 0; m < 3; m++) {
+            if (ws + m == j) dg = S[m];
+            else s += S[m] * x[((size_t)(ws + m) * nxg + i) * nzg + k];
+        }
+    }
+    x[q] = (b[q] - s) / dg;
+}
+
+// LDS-tiled residual r = b - L x (16 x 16 columns per block, halo 2 on domain ends as the matvec).
+template <typename T, int NZ>
+__global__ __launch_bounds__(256) void k_mg_residual_t(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
+    constexpr int TX = 16, TY = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1;
+    __shared__ T sX[WX * WY * NZP];
+    const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
+    const int tyd = d3 ? TY : 1;
+    const int i0 = d3 ? (int)blockIdx.x * TX : (int)blockIdx.x * (TX * TY), j0 = d3 ? (int)blockIdx.y * TY : 0;
+    const int tw = d3 ? TX : TX * TY;        // columns per tile row
+    const int lox = cmax(cmin(i0 - R, nxg - 4), 0);
+    const int loy = d3 ? cmax(cmin(j0 - R, nyg - 4), 0) : 0;
+    const int wxmax = d3 ? WX : WX * WY;     // 2D: the tile is one long row
+    const int wcols = cmin(tw + 2 * R, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
+    (void)wxmax;
+    const int rowstride = d3 ? WX : (TX * TY + 2 * R);
+    for (int rr = 0; rr < wrows; rr++) {
+        const T *src = x + ((size_t)(loy + rr) * nxg + lox) * NZ;
+        for (int e = threadIdx.x; e < wcols * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            sX[(rr * rowstride + c) * NZP + k] = src[e];
+        }
+    }
+    __syncthreads();
+    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * rowstride + (ii - lox)) * NZP + k]; };
+    const int tx = d3 ? (int)threadIdx.x % TX : (int)threadIdx.x, ty = d3 ? (int)threadIdx.x / TX : 0;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = i < nxg && j < nyg && (d3 || ty == 0);
+    T o[NZ];
+    if (active) {
+        const int p = i + nxg * j;
+        const T *bc = b + (size_t)p * NZ;
+        const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
+        if (wx && wy) {
+#pragma unroll
+            for (int k = 0; k < NZ; k++) o[k] = bc[k] - X(k, i, j);
+        } else if (wx || wy) {
+            o[0] = bc[0] - X(0, i, j);
+            if (wx) {
+                const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
+                const T *S = L.sx + (ib * 2) * 3;
+#pragma unroll
+                for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j));
+            } else {
+                const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
+                const T *S = L.sy + (jb * 2) * 3;
+#pragma unroll
+                for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2));
+            }
+        } else {
+            const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
+            const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
+            const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
+            T xc[NZ], dxl[NZ], dyl[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; k++) {
+                xc[k] = X(k, i, j);
+                dxl[k] = (SX1[0] * X(k, wsx, j) + SX1[1] * X(k, wsx + 1, j)) + SX1[2] * X(k, wsx + 2, j);
+                dyl[k] = d3 ? (SY1[0] * X(k, i, wsy) + SY1[1] * X(k, i, wsy + 1)) + SY1[2] * X(k, i, wsy + 2) : T(0);
+            }
+            {
+                int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
+                if (sh) { const T *S = L.sx + ((i - sh) * 2) * 3; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
+                if (d3) {
+                    sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
+                    if (sh) { const T *S = L.sy + ((j - sh) * 2) * 3; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
+                }
+            }
+            o[NZ - 1] = bc[NZ - 1] - xc[NZ - 1];
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
+                const T dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
+                const T dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
+                const T dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
+                const T dyy = d3 ? (SY2[0] * X(k, i, wsy) + SY2[1] * X(k, i, wsy + 1)) + SY2[2] * X(k, i, wsy + 2) : T(0);
+                const T dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
+                const T dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
+                T s2, s3, s4, s5;
+                low_coefs(L, p, k, s2, s3, s4, s5);
+                o[k] = bc[k] - (dxx + dyy + s3 * dz + T(2) * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz);
+            }
+            {
+                const T *Z1 = L.sz + (1 * 2) * 3;
+                const T dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
+                T s2, s3, s4, s5;
+                low_coefs(L, p, 1, s2, s3, s4, s5);
+                const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
+                o[0] = bc[0] - ((s5 + hx * s2 + hy * s4) * dz + hx * dxl[1] + hy * dyl[1]);
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+        T *dst = sX + ((j - loy) * rowstride + (i - lox)) * NZP;
+#pragma unroll
+        for (int k = 0; k < NZ; k++) dst[k] = o[k];
+    }
+    __syncthreads();
+    const int ie = cmin(i0 + tw, nxg), je = d3 ? cmin(j0 + TY, nyg) : 1;
+    for (int rr = 0; rr < je - j0; rr++) {
+        T *dst = r + ((size_t)(j0 + rr) * nxg + i0) * NZ;
+        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            dst[e] = sX[((j0 + rr - loy) * rowstride + (i0 + c - lox)) * NZP + k];
+        }
+    }
 }
 
 // Wall ghost points: Neumann rows relaxed pointwise, identity rows set (GaussSeidelSpecialPoints.f90 role).

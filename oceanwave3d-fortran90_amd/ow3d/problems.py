@@ -151,4 +151,14 @@ def workload(name, seed=0):
         dt = 0.5 * dx / info["cp"]
         cfg = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly)
         return cfg, bath, E, P, dt, sponge_zones(nx, ny, lx, ly, 5 * lam)
+    if name == "shoal257":         # same family as shoal513 at the same resolution, a quarter of the area
+        nx = ny = 257; nz = 9
+        kp = 1.5; lam = 2 * np.pi / kp
+        lx = ly = 16 * lam
+        bath = gaussian_shoal(nx, ny, lx, ly)
+        E, P, info = random_waves(nx, ny, lx, ly, hs=0.02, kph=1.5, seed=seed)
+        dx = lx / (nx - 1)
+        dt = 0.5 * dx / info["cp"]
+        cfg = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly)
+        return cfg, bath, E, P, dt, sponge_zones(nx, ny, lx, ly, 5 * lam)
     raise KeyError(name)

@@ -73,6 +73,11 @@ def lib():
                               ctypes.POINTER(ctypes.POINTER(I)), ctypes.POINTER(ctypes.POINTER(I)),
                               ctypes.POINTER(D)]
         L.owo_fornberg.argtypes = [ctypes.c_double, D, I, I, D]
+        L.owo_level_grid.restype = I
+        L.owo_level_grid.argtypes = [P, I, ctypes.POINTER(I), D, D, D, D, D, D, D, D]
+        L.owo_level_ghosts.restype = I
+        L.owo_level_ghosts.argtypes = [P, I, ctypes.POINTER(I)]
+        L.owo_gs_once.argtypes = [P, I, D, D]
         _lib = L
     return _lib
 
@@ -230,6 +235,30 @@ class OracleTank:
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.array(x, dtype=np.float64, copy=True)
         lib().owo_smooth(self.h, level, _p(b), _p(x), sweeps)
+        return x
+
+    def level_grid(self, level):
+        d = (ctypes.c_int * 6)()
+        if lib().owo_level_grid(self.h, level, d, *([None] * 8)) != 0:
+            raise IndexError(level)
+        nx, ny, nz, nxg, nyg, nzg = list(d)
+        out = dict(nx=nx, ny=ny, nz=nz, nxg=nxg, nyg=nyg, nzg=nzg, n=nxg * nyg * nzg,
+                   x=np.zeros(nxg), y=np.zeros(nyg), z=np.zeros(nzg))
+        for k in ("h", "hx", "hy", "hxx", "hyy"):
+            out[k] = np.zeros((nyg, nxg))
+        lib().owo_level_grid(self.h, level, d, *[_p(out[k]) for k in ("x", "y", "z", "h", "hx", "hy", "hxx", "hyy")])
+        return out
+
+    def level_ghosts(self, level):
+        n = lib().owo_level_ghosts(self.h, level, None)
+        out = np.zeros(n, dtype=np.int32)
+        lib().owo_level_ghosts(self.h, level, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out
+
+    def smooth_gs_once(self, level, b, x):
+        b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        x = np.array(x, dtype=np.float64, copy=True).ravel()
+        lib().owo_gs_once(self.h, level, _p(b), _p(x))
         return x
 
     def csr(self, level):

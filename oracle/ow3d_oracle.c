@@ -1246,6 +1246,32 @@ void owo_prolong(owo_tank *t, int fine_level, const double *ec, double *ef) {
 void owo_smooth(owo_tank *t, int level, const double *b, double *x, int sweeps) {
     smooth_sweeps(t, &t->lev[level], x, b, sweeps, 1);
 }
+/* Level grid as built by mg_setup: dims6 = nx, ny, nz, nxg, nyg, nzg; coordinate and depth arrays. */
+int owo_level_grid(const owo_tank *t, int level, int *dims6, double *x, double *y, double *z, double *h,
+                   double *hx, double *hy, double *hxx, double *hyy) {
+    if (level < 0 || level >= t->nlev) return -1;
+    const grid_t *g = &t->lev[level].g;
+    dims6[0] = g->nx; dims6[1] = g->ny; dims6[2] = g->nz; dims6[3] = g->nxg; dims6[4] = g->nyg; dims6[5] = g->nzg;
+    if (x) memcpy(x, g->x, g->nxg * 8);
+    if (y) memcpy(y, g->y, g->nyg * 8);
+    if (z) memcpy(z, g->z, g->nzg * 8);
+    if (h) memcpy(h, g->h, g->nxy * 8);
+    if (hx) memcpy(hx, g->hx, g->nxy * 8);
+    if (hy) memcpy(hy, g->hy, g->nxy * 8);
+    if (hxx) memcpy(hxx, g->hxx, g->nxy * 8);
+    if (hyy) memcpy(hyy, g->hyy, g->nxy * 8);
+    return 0;
+}
+int owo_level_ghosts(const owo_tank *t, int level, int *out) {
+    const level_t *L = &t->lev[level];
+    if (out) memcpy(out, L->ighost, L->nghost * sizeof(int));
+    return L->nghost;
+}
+/* One GaussSeidel sweep followed by one GaussSeidelSpecialPoints pass (reference mode). */
+void owo_gs_once(owo_tank *t, int level, const double *b, double *x) {
+    gs_csr(&t->lev[level], x, b);
+    gs_special(&t->lev[level], x, b);
+}
 int owo_csr(owo_tank *t, int level, int *nrow, int *nnz, const int **rowptr, const int **col, const double **val) {
     level_t *L = &t->lev[level];
     if (!L->rp) return -1;

@@ -73,6 +73,10 @@ void owo_prolong(owo_tank *t, int fine_level, const double *ec, double *ef);
 void owo_smooth(owo_tank *t, int level, const double *b, double *x, int sweeps);
 int  owo_csr(owo_tank *t, int level, int *nrow, int *nnz, const int **rowptr, const int **col, const double **val);
 void owo_fornberg(double z, const double *x, int n, int m, double *c);
+int  owo_level_grid(const owo_tank *t, int level, int *dims6, double *x, double *y, double *z, double *h,
+                    double *hx, double *hy, double *hxx, double *hyy);
+int  owo_level_ghosts(const owo_tank *t, int level, int *out);
+void owo_gs_once(owo_tank *t, int level, const double *b, double *x);
 
 #ifdef __cplusplus
 }

@@ -4,7 +4,8 @@
 # HSL MA41, LAPACK: iterative_solution, dc, MultiGridSolver, CleanSparseMatrixCOO, ...) are not built.
 REFOBJ := _ref/obj
 REF_MODS := variabledefs/precision.f90 variabledefs/constants.f90 variabledefs/datatypes.f90 \
-            variabledefs/HSL_LU.f90 variabledefs/OFmodule.f90 variabledefs/globalvariables.f90
+            variabledefs/HSL_LU.f90 variabledefs/OFmodule.f90 variabledefs/globalvariables.f90 \
+            multigrid/MGLevels.f90
 REF_SRCS := functions/BuildLinearSystem.f90 functions/DiffXEven.f90 functions/DiffYEven.f90 \
             functions/DiffZArbitrary.f90 functions/DiffXEven_CD.f90 functions/DiffYEven_CD.f90 \
             functions/DiffZArbitrary_CD.f90 functions/ConstructTableCrossDerivatives.f90 \

@@ -116,10 +116,16 @@ def test_solve_parity(oracle_mod, nx, ny, nz, shoal, solver):
     x0 = np.zeros(T_o.shape3)
     xo, sto, ito, _ = T_o.solve(b, x0)
     xg, stg, itg, _ = T_g.solve(b, x0)
-    ref = T_o.matvec(xg)
-    assert rel(ref, b) < 1e-8
-    assert rel(xg, xo) < 1e-8, (rel(xg, xo), ito, itg)
-    assert abs(ito - itg) <= 2
+    if solver == 1:
+        # GMRES converges: both solutions solve the system to the tolerance
+        assert rel(T_o.matvec(xg), b) < 1e-8
+        assert rel(xg, xo) < 1e-8, (rel(xg, xo), ito, itg)
+        assert abs(ito - itg) <= 2
+    else:
+        # defect correction contracts by ~0.5 per sweep and stops at the matvec budget: the same
+        # number of identical sweeps must give the same iterate
+        assert ito == itg
+        assert rel(xg, xo) < 1e-9, (rel(xg, xo), ito, itg)
 
 
 @pytest.mark.parametrize("nx,ny,nz,shoal,sponge", [(33, 33, 9, True, True), (65, 1, 9, False, False)])
