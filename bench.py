@@ -77,12 +77,18 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
+                    help="only run K high-order matvec launches of the workload and a 256 MiB device copy "
+                         "(the calibration kernel for HBM counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
     import ow3d
     from ow3d import problems
+    if args.profile_matvec:
+        profile_matvec(ow3d, problems, args.profile_matvec)
+        return
 
     results = {}
     for label, prec in (("fp64", 0), ("mixed", 1)):
@@ -134,6 +140,22 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
+
+
+def profile_matvec(ow3d, problems, k):
+    """K matvec launches on the 513x513x9 shoal tank plus one 256 MiB device-to-device copy, for
+    rocprofv3 counter runs: the copy's FETCH_SIZE / WRITE_SIZE per byte calibrate the matvec's."""
+    import torch
+    T, cfg, dt = make_tank(ow3d, problems, 1)
+    ms = T.time_matvec(k)
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    n = T.n
+    print(json.dumps({"matvec_launches": k, "avg_ms": ms, "n": n, "ncol": T.nxg * T.nyg,
+                      "algorithmic_bytes": 16 * n + 40 * T.nxg * T.nyg, "copy_bytes": 2 * src.numel() * 8}), flush=True)
+    T.close()
 
 
 def make_tank_state(problems):

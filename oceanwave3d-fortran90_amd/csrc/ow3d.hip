@@ -69,7 +69,6 @@ struct ow3d_tank {
     ow3d_config cfg{};
     Grid g;
     hipStream_t s = nullptr;
-    bool flat = true;
     int A = 3;
     // high-order operator
     DevBuf<double> sx, sy, sz, zs, bath, h, cf;
@@ -109,19 +108,23 @@ struct ow3d_tank {
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+template <int NZ, int A, int TY>
+constexpr size_t mv3_lds() {
+    constexpr int R = A + 1, RA = 2 * A + 1;
+    return ((size_t)(16 + 2 * R) * (TY + 2 * R) * NZ + 2 * 16 * TY * NZ + (16 + 2) * 2 * RA + (TY + 2) * 2 * RA + NZ * 2 * RA + NZ) * 8;
+}
 template <int NZ, int A>
 int launch_mv3(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
-    constexpr int R = A + 1, RA = 2 * A + 1;
-    constexpr size_t lds16 = ((size_t)NZ * (((16 + 2 * R) * (16 + 2 * R)) | 1) + 36 * RA) * 8;
-    constexpr int TY = lds16 <= 65536 ? 16 : 8;
+    constexpr int TY = mv3_lds<NZ, A, 8>() <= 65536 ? 8 : 4;
+    static_assert(mv3_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
     dim3 grid((t->g.nxg + 15) / 16, (t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3<NZ, A, TY>), grid, dim3(16 * TY), 0, s, t->mv, phi, y);
+    hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), grid, dim3(256), 0, s, t->mv, phi, y);
     return 0;
 }
 template <int NZ, int A>
 int launch_mv2(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
-    dim3 grid((t->g.nxg + 255) / 256);
-    hipLaunchKernelGGL((k_matvec2<NZ, A>), grid, dim3(256), 0, s, t->mv, phi, y);
+    dim3 grid((t->g.nxg + 63) / 64);
+    hipLaunchKernelGGL((k_matvec2p<NZ, A>), grid, dim3(256), 0, s, t->mv, phi, y);
     return 0;
 }
 
@@ -224,7 +227,7 @@ void dense_inverse(const std::vector<double> &A, int n, std::vector<double> &Ain
 }
 
 // Host evaluation of the level operator applied to a unit vector (coarsest level assembly).
-// Same row semantics as k_mg_residual, in double.
+// Same row semantics as k_mg_residual_t, in double.
 void low_apply_host(const Grid &g, const std::vector<double> &x, std::vector<double> &y) {
     const int nxg = g.nxg, nyg = g.nyg, nzg = g.nzg, nxy = g.nxy;
     const bool d3 = g.ny > 1;
@@ -598,7 +601,7 @@ PlaneArgs plane_args(ow3d_tank *t, double *E, double *P) {
     PlaneArgs a{};
     a.nxg = t->g.nxg; a.nyg = t->g.nyg; a.nxy = t->g.nxy; a.gx = t->g.gx; a.gy = t->g.gy;
     a.a = t->g.a; a.b = t->g.b; a.nonlinear = t->cfg.nonlinear;
-    a.sx = t->sx.p; a.sy = t->sy.p; a.h = t->h.p;
+    a.sx = t->sx.p; a.sy = t->sy.p; a.h = t->h.p; a.bath = t->bath.p;
     a.E = E; a.P = P; a.Ex = t->Ex.p; a.Exx = t->Exx.p; a.Ey = t->Ey.p; a.Eyy = t->Eyy.p; a.Px = t->Px.p; a.Py = t->Py.p;
     a.cf = t->cf.p;
     return a;
@@ -667,9 +670,6 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (hxx) std::copy(hxx, hxx + nxy, g.hxx.begin());
     if (hyy) std::copy(hyy, hyy + nxy, g.hyy.begin());
     grid_stencils(g);
-    t->flat = true;
-    for (size_t p = 0; p < nxy; p++)
-        if (g.hx[p] != 0.0 || (c.ny > 1 && g.hy[p] != 0.0) || g.hxx[p] != 0.0 || (c.ny > 1 && g.hyy[p] != 0.0)) { t->flat = false; break; }
     HIPCHK(hipStreamCreate(&t->s));
     HIPCHK(hipEventCreate(&t->ev0));
     HIPCHK(hipEventCreate(&t->ev1));
@@ -688,8 +688,15 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
         bath[p] = g.hx[p]; bath[nxy + p] = c.ny > 1 ? g.hy[p] : 0.0; bath[2 * nxy + p] = g.hxx[p] + (c.ny > 1 ? g.hyy[p] : 0.0);
     }
     HIPCHK(upd(t->bath, bath));
-    std::vector<double> cf0((size_t)4 * nxy, 0.0);
-    for (size_t p = 0; p < nxy; p++) cf0[p] = g.h[p];
+    // sigma-coefficient planes of the still surface (the same expressions the surface kernels evaluate)
+    std::vector<double> cf0((size_t)7 * nxy, 0.0);
+    for (size_t p = 0; p < nxy; p++) {
+        const double id = 1.0 / g.h[p], hx = bath[p], hy = bath[nxy + p], hs = bath[2 * nxy + p];
+        const double H2 = hx * hx + hy * hy;
+        cf0[p] = hx * id; cf0[nxy + p] = -hx * id; cf0[2 * nxy + p] = hy * id; cf0[3 * nxy + p] = -hy * id;
+        cf0[4 * nxy + p] = (-2.0 * H2) * (id * id) + hs * id; cf0[5 * nxy + p] = (2.0 * H2) * (id * id) - hs * id;
+        cf0[6 * nxy + p] = id;
+    }
     HIPCHK(upd(t->cf, cf0));
     for (DevBuf<double> *b : {&t->E, &t->P, &t->W, &t->Ex, &t->Exx, &t->Ey, &t->Eyy, &t->Px, &t->Py, &t->Et, &t->Pt, &t->Wt, &t->KE, &t->KP}) {
         HIPCHK(b->alloc(nxy));
@@ -705,7 +712,6 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     HIPCHK(t->hv.alloc(MAXK));
     t->mv.nxg = g.nxg; t->mv.nyg = g.nyg; t->mv.nxy = g.nxy;
     t->mv.sx = t->sx.p; t->mv.sy = t->sy.p; t->mv.sz = t->sz.p; t->mv.zs = t->zs.p; t->mv.cf = t->cf.p;
-    t->mv.bath = t->flat ? nullptr : t->bath.p;
     if (c.precond == 3) {
         if (!mg_levels(g, c.mg_maxgrids, t->levels, g_err)) return -1;
         t->nlev = (int)t->levels.size();

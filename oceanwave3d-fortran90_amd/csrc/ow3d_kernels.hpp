@@ -49,279 +49,264 @@ struct MvArgs {
     int nxg, nyg, nxy;
     const double *sx, *sy, *sz;  // stencil rows [N][2][2A+1]
     const double *zs;            // sigma of each level (0 on the ghost level)
-    const double *cf;            // 4 planes: d = eta+h, eta_x, eta_y, eta_xx+eta_yy
-    const double *bath;          // 3 planes: h_x, h_y, h_xx+h_yy (nullptr: flat bottom)
+    const double *cf;            // 7 planes: sigma coefficients affine in sigma, s2 = A2 + sigma*B2, s4 = A4 + sigma*B4,
+                                 // s3 = A3 + sigma*B3, s5 = S5; planes (A2, B2, A4, B4, A3, B3, S5)
 };
 
-// 3D: block = 16 x TY columns, one thread per sigma column.  The tile is kept level-major in LDS
-// (plane stride PS odd, so lanes of a wave touch consecutive words on every access); the stencil
-// rows of the tile's columns and rows are staged in LDS as well.
-template <int NZ, int A, int TY>
-__global__ __launch_bounds__(16 * TY) void k_matvec3(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
-    constexpr int TX = 16, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R, PS = (WX * WY) | 1;
-    __shared__ double sP[NZ * PS];
+// Per-column coefficient planes from the surface and the bottom: DetermineTransformationConstantsArray.f90
+// (lines 190-201) written as functions affine in sigma, evaluated once per column and stage.
+__device__ __forceinline__ void column_coefs(double d, double Ex, double Ey, double Es, double hx, double hy, double hs,
+                                             double *cf, int nxy, int p) {
+    const double id = 1.0 / d, id2 = id * id;
+    const double H2 = hx * hx + hy * hy, C = Ex * hx + Ey * hy, E2 = Ex * Ex + Ey * Ey;
+    cf[p] = hx * id;
+    cf[nxy + p] = -(hx + Ex) * id;
+    cf[2 * nxy + p] = hy * id;
+    cf[3 * nxy + p] = -(hy + Ey) * id;
+    cf[4 * nxy + p] = (-2.0 * H2 - 2.0 * C) * id2 + hs * id;
+    cf[5 * nxy + p] = (2.0 * H2 + 4.0 * C + 2.0 * E2) * id2 - (hs + Es) * id;
+    cf[6 * nxy + p] = id;
+}
+
+// 3D, one thread per grid point: block = 16 x TY columns x NZ levels, NT threads.  The tile is
+// stored k-fastest in LDS exactly as in HBM (row segments are straight copies), and threads are mapped
+// k-fastest too, so every LDS access of a wave and every global load / store is contiguous.
+// Phase 1 forms the x- and y-derivatives of every tile point (the first passes of the mixed
+// derivatives go to LDS); phase 2 the sigma derivatives, the mixed derivatives and the row.
+template <int NZ, int A, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    constexpr int TX = 16, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R, NC = TX * TY, NP = NC * NZ;
+    constexpr int PER = (NP + NT - 1) / NT;
+    __shared__ double sP[WX * WY * NZ];
+    __shared__ double sDx[NP];
+    __shared__ double sDy[NP];
     __shared__ double sSX[(TX + 2) * 2 * RA];
     __shared__ double sSY[(TY + 2) * 2 * RA];
+    __shared__ double sSZ[NZ * 2 * RA];
+    __shared__ double sZS[NZ];
     const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
     const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
     for (int jj = 0; jj < wy; jj++) {
         const double *src = phi + ((size_t)(loy + jj) * nxg + lox) * NZ;
-        for (int e = threadIdx.x; e < wx * NZ; e += blockDim.x) {
-            const int ii = e / NZ, k = e - ii * NZ;
-            sP[k * PS + jj * WX + ii] = src[e];
-        }
+        double *dst = sP + jj * WX * NZ;
+        for (int e = threadIdx.x; e < wx * NZ; e += NT) dst[e] = src[e];
     }
-    for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += blockDim.x) {
+    for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
         const int r = e / (2 * RA), ii = i0 - 1 + r;
         if (ii >= 0 && ii < nxg) sSX[e] = g.sx[(size_t)ii * 2 * RA + (e - r * 2 * RA)];
     }
-    for (int e = threadIdx.x; e < (TY + 2) * 2 * RA; e += blockDim.x) {
+    for (int e = threadIdx.x; e < (TY + 2) * 2 * RA; e += NT) {
         const int r = e / (2 * RA), jj = j0 - 1 + r;
         if (jj >= 0 && jj < nyg) sSY[e] = g.sy[(size_t)jj * 2 * RA + (e - r * 2 * RA)];
     }
+    for (int e = threadIdx.x; e < NZ * 2 * RA; e += NT) sSZ[e] = g.sz[e];
+    if (threadIdx.x < NZ) sZS[threadIdx.x] = g.zs[threadIdx.x];
     __syncthreads();
-    auto P = [&](int k, int ii, int jj) -> double { return sP[k * PS + (jj - loy) * WX + (ii - lox)]; };
+    auto P = [&](int k, int ii, int jj) -> double { return sP[((jj - loy) * WX + (ii - lox)) * NZ + k]; };
     auto SX = [&](int ii) -> const double * { return sSX + (ii - (i0 - 1)) * 2 * RA; };
     auto SY = [&](int jj) -> const double * { return sSY + (jj - (j0 - 1)) * 2 * RA; };
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = (i < nxg) && (j < nyg);
-    double o[NZ];
-    if (active) {
-        const bool wxl = (i == 0 || i == nxg - 1), wyl = (j == 0 || j == nyg - 1);
-        if (wxl && wyl) {
+    double acc[PER];
 #pragma unroll
-            for (int k = 0; k < NZ; k++) o[k] = P(k, i, j);
+    for (int r = 0; r < PER; r++) {
+        acc[r] = 0.0;
+        const int pt = threadIdx.x + r * NT;
+        if (pt >= NP) break;
+        const int col = pt / NZ, k = pt - col * NZ;
+        const int i = i0 + col % TX, j = j0 + col / TX;
+        if (i >= nxg || j >= nyg || i == 0 || i == nxg - 1 || j == 0 || j == nyg - 1) continue;
+        const int wsx = cclamp(i - A, 0, nxg - RA), wsy = cclamp(j - A, 0, nyg - RA);
+        double d1 = 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
+        {
+            const double *S1 = SX(i), *S2 = S1 + RA;
+#pragma unroll
+            for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += S1[m] * v; d2 += S2[m] * v; }
+        }
+        {
+            const double *S1 = SY(j), *S2 = S1 + RA;
+#pragma unroll
+            for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); e1 += S1[m] * v; e2 += S2[m] * v; }
+        }
+        if (k == 0) {  // corner-corrected first pass on the sigma ghost level (ConstructTableCrossDerivatives.f90:173-186)
+            int sh = 0;
+            if (wsx == 0) sh = 1;
+            if (wsx + RA - 1 == nxg - 1) sh = -1;
+            if (sh != 0) {
+                const double *S = SX(i - sh);
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) t += S[m] * P(0, wsx + sh + m, j);
+                d1 = t;
+            }
+            sh = 0;
+            if (wsy == 0) sh = 1;
+            if (wsy + RA - 1 == nyg - 1) sh = -1;
+            if (sh != 0) {
+                const double *S = SY(j - sh);
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) t += S[m] * P(0, i, wsy + sh + m);
+                e1 = t;
+            }
+        }
+        sDx[pt] = d1;
+        sDy[pt] = e1;
+        acc[r] = d2 + e2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int pt = threadIdx.x + r * NT;
+        if (pt >= NP) break;
+        const int col = pt / NZ, k = pt - col * NZ;
+        const int i = i0 + col % TX, j = j0 + col / TX;
+        if (i >= nxg || j >= nyg) continue;
+        const bool wxl = (i == 0 || i == nxg - 1), wyl = (j == 0 || j == nyg - 1);
+        double o;
+        if ((wxl && wyl) || ((wxl || wyl) && k == 0) || (!wxl && !wyl && k == NZ - 1)) {
+            o = P(k, i, j);
         } else if (wxl) {
             const int ib = (i == 0) ? 1 : nxg - 2, ws = cclamp(ib - A, 0, nxg - RA);
             const double *S = SX(ib);
-            o[0] = P(0, i, j);
+            o = 0.0;
 #pragma unroll
-            for (int k = 1; k < NZ; k++) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < RA; m++) s += S[m] * P(k, ws + m, j);
-                o[k] = s;
-            }
+            for (int m = 0; m < RA; m++) o += S[m] * P(k, ws + m, j);
         } else if (wyl) {
             const int jb = (j == 0) ? 1 : nyg - 2, ws = cclamp(jb - A, 0, nyg - RA);
             const double *S = SY(jb);
-            o[0] = P(0, i, j);
+            o = 0.0;
 #pragma unroll
-            for (int k = 1; k < NZ; k++) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < RA; m++) s += S[m] * P(k, i, ws + m);
-                o[k] = s;
-            }
+            for (int m = 0; m < RA; m++) o += S[m] * P(k, i, ws + m);
         } else {
-            double dx[NZ], dy[NZ];
-            const int wsx = cclamp(i - A, 0, nxg - RA), wsy = cclamp(j - A, 0, nyg - RA);
-            {
-                const double *S1 = SX(i), *S2 = S1 + RA;
-#pragma unroll
-                for (int k = 0; k < NZ; k++) {
-                    double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += S1[m] * v; d2 += S2[m] * v; }
-                    dx[k] = d1; o[k] = d2;
-                }
-            }
-            {
-                const double *S1 = SY(j), *S2 = S1 + RA;
-#pragma unroll
-                for (int k = 0; k < NZ; k++) {
-                    double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); d1 += S1[m] * v; d2 += S2[m] * v; }
-                    dy[k] = d1; o[k] = o[k] + d2;
-                }
-            }
-            // corner-corrected first pass on the sigma ghost level (ConstructTableCrossDerivatives.f90:173-186)
-            double dxc0 = dx[0], dyc0 = dy[0];
-            {
-                int sh = 0;
-                if (wsx == 0) sh = 1;
-                if (wsx + RA - 1 == nxg - 1) sh = -1;
-                if (sh != 0) {
-                    const double *S = SX(i - sh);
-                    double s = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) s += S[m] * P(0, wsx + sh + m, j);
-                    dxc0 = s;
-                }
-                sh = 0;
-                if (wsy == 0) sh = 1;
-                if (wsy + RA - 1 == nyg - 1) sh = -1;
-                if (sh != 0) {
-                    const double *S = SY(j - sh);
-                    double s = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) s += S[m] * P(0, i, wsy + sh + m);
-                    dyc0 = s;
-                }
-            }
             const int p = i + nxg * j;
-            const double d = g.cf[p], Ex = g.cf[nxy + p], Ey = g.cf[2 * nxy + p], Es = g.cf[3 * nxy + p];
-            double hx = 0.0, hy = 0.0, hs = 0.0;
-            if (g.bath) { hx = g.bath[p]; hy = g.bath[nxy + p]; hs = g.bath[2 * nxy + p]; }
-            double obot;
-            {
+            const double A2 = g.cf[p], B2 = g.cf[nxy + p], A4 = g.cf[2 * nxy + p], B4 = g.cf[3 * nxy + p];
+            const double S5 = g.cf[6 * nxy + p];
+            const double *dxc = sDx + col * NZ, *dyc = sDy + col * NZ;
+            if (k == 0) {
                 const int wz = cclamp(1 - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (size_t)(1 * 2) * RA;
+                const double *Z1 = sSZ + (1 * 2) * RA;
                 double dz = 0.0;
 #pragma unroll
                 for (int m = 0; m < RA; m++) dz += Z1[m] * P(wz + m, i, j);
-                double s2, s3, s4, s5;
-                sigma_coef3<double>(g.zs[1], d, hx, hy, hs, Ex, Ey, Es, s2, s3, s4, s5);
-                obot = (s5 + hx * s2 + hy * s4) * dz + hx * dx[1] + hy * dy[1];
-            }
-#pragma unroll
-            for (int k = 1; k < NZ - 1; k++) {
+                // h_x = A2 * d and h_y = A4 * d with d = 1 / S5: no bottom-slope planes are read
+                const double sg = sZS[1], d = 1.0 / S5, hx = A2 * d, hy = A4 * d;
+                o = (S5 + hx * (A2 + sg * B2) + hy * (A4 + sg * B4)) * dz + hx * dxc[1] + hy * dyc[1];
+            } else {
+                const double A3 = g.cf[4 * nxy + p], B3 = g.cf[5 * nxy + p];
                 const int wz = cclamp(k - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (size_t)(k * 2) * RA, *Z2 = Z1 + RA;
+                const double *Z1 = sSZ + (k * 2) * RA, *Z2 = Z1 + RA;
                 double dz = 0.0, dzz = 0.0, dxz = 0.0, dyz = 0.0;
 #pragma unroll
                 for (int m = 0; m < RA; m++) {
                     const double z1 = Z1[m], cv = P(wz + m, i, j);
                     dz += z1 * cv;
                     dzz += Z2[m] * cv;
-                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
-                    dyz += z1 * ((wz + m == 0) ? dyc0 : dy[wz + m]);
+                    dxz += z1 * dxc[wz + m];
+                    dyz += z1 * dyc[wz + m];
                 }
-                double s2, s3, s4, s5;
-                sigma_coef3<double>(g.zs[k], d, hx, hy, hs, Ex, Ey, Es, s2, s3, s4, s5);
-                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz;
-            }
-            o[0] = obot;
-            o[NZ - 1] = P(NZ - 1, i, j);
-        }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int k = 0; k < NZ; k++) sP[k * PS + (j - loy) * WX + (i - lox)] = o[k];
-    }
-    __syncthreads();
-    {
-        const int ie = cmin(i0 + TX, nxg), je = cmin(j0 + TY, nyg);
-        for (int jj = j0; jj < je; jj++) {
-            double *dst = out + ((size_t)jj * nxg + i0) * NZ;
-            for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
-                const int ii = e / NZ, k = e - ii * NZ;
-                dst[e] = sP[k * PS + (jj - loy) * WX + (i0 + ii - lox)];
+                const double sg = sZS[k];
+                const double s2 = A2 + sg * B2, s4 = A4 + sg * B4, s3 = A3 + sg * B3;
+                o = acc[r] + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + S5 * S5) * dzz;
             }
         }
+        out[((size_t)j * nxg + i) * NZ + k] = o;
     }
 }
 
-// x-z problems (Ny == 1): block = 256 columns of the single row.
+// x-z problems (Ny == 1), one thread per grid point: block = 64 columns x NZ levels.
 template <int NZ, int A>
-__global__ __launch_bounds__(256) void k_matvec2(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
-    constexpr int TX = 256, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, NZP = NZ | 1;
-    __shared__ double sP[WX * NZP];
+__global__ __launch_bounds__(256) void k_matvec2p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    constexpr int NT = 256, TX = 64, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, NP = TX * NZ, PER = (NP + NT - 1) / NT;
+    __shared__ double sP[WX * NZ];
+    __shared__ double sDx[NP];
     const int nxg = g.nxg, nxy = g.nxy;
     const int i0 = blockIdx.x * TX;
     const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox);
-    for (int f = threadIdx.x; f < wx * NZ; f += blockDim.x) {
-        const int ii = f / NZ, k = f - ii * NZ;
-        sP[ii * NZP + k] = phi[(size_t)lox * NZ + f];
+    for (int e = threadIdx.x; e < wx * NZ; e += NT) sP[e] = phi[(size_t)lox * NZ + e];
+    __syncthreads();
+    auto P = [&](int k, int ii) -> double { return sP[(ii - lox) * NZ + k]; };
+    double acc[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        acc[r] = 0.0;
+        const int pt = threadIdx.x + r * NT;
+        if (pt >= NP) break;
+        const int col = pt / NZ, k = pt - col * NZ;
+        const int i = i0 + col;
+        if (i >= nxg || i == 0 || i == nxg - 1) continue;
+        const int wsx = cclamp(i - A, 0, nxg - RA);
+        const double *S1 = g.sx + (size_t)(i * 2) * RA, *S2 = S1 + RA;
+        double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m); d1 += S1[m] * v; d2 += S2[m] * v; }
+        if (k == 0) {
+            int sh = 0;
+            if (wsx == 0) sh = 1;
+            if (wsx + RA - 1 == nxg - 1) sh = -1;
+            if (sh != 0) {
+                const double *S = g.sx + (size_t)((i - sh) * 2) * RA;
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) t += S[m] * P(0, wsx + sh + m);
+                d1 = t;
+            }
+        }
+        sDx[pt] = d1;
+        acc[r] = d2;
     }
     __syncthreads();
-    auto P = [&](int k, int ii) -> double { return sP[(ii - lox) * NZP + k]; };
-    const int i = i0 + threadIdx.x;
-    const bool active = i < nxg;
-    double o[NZ];
-    if (active) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int pt = threadIdx.x + r * NT;
+        if (pt >= NP) break;
+        const int col = pt / NZ, k = pt - col * NZ;
+        const int i = i0 + col;
+        if (i >= nxg) continue;
+        double o;
         if (i == 0 || i == nxg - 1) {
-            const int ib = (i == 0) ? 1 : nxg - 2, ws = cclamp(ib - A, 0, nxg - RA);
-            const double *S = g.sx + (size_t)(ib * 2) * RA;
-            o[0] = P(0, i);
+            if (k == 0) o = P(0, i);
+            else {
+                const int ib = (i == 0) ? 1 : nxg - 2, ws = cclamp(ib - A, 0, nxg - RA);
+                const double *S = g.sx + (size_t)(ib * 2) * RA;
+                o = 0.0;
 #pragma unroll
-            for (int k = 1; k < NZ; k++) {
-                double s = 0.0;
-#pragma unroll
-                for (int m = 0; m < RA; m++) s += S[m] * P(k, ws + m);
-                o[k] = s;
+                for (int m = 0; m < RA; m++) o += S[m] * P(k, ws + m);
             }
+        } else if (k == NZ - 1) {
+            o = P(k, i);
         } else {
-            double c[NZ], dx[NZ], axx[NZ];
-#pragma unroll
-            for (int k = 0; k < NZ; k++) c[k] = P(k, i);
-            const int wsx = cclamp(i - A, 0, nxg - RA);
-            {
-                const double *S1 = g.sx + (size_t)(i * 2) * RA, *S2 = S1 + RA;
-                double a1[RA], a2[RA];
-#pragma unroll
-                for (int m = 0; m < RA; m++) { a1[m] = S1[m]; a2[m] = S2[m]; }
-#pragma unroll
-                for (int k = 0; k < NZ; k++) {
-                    double d1 = 0.0, d2 = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m); d1 += a1[m] * v; d2 += a2[m] * v; }
-                    dx[k] = d1; axx[k] = d2;
-                }
-            }
-            double dxc0 = dx[0];
-            {
-                int sh = 0;
-                if (wsx == 0) sh = 1;
-                if (wsx + RA - 1 == nxg - 1) sh = -1;
-                if (sh != 0) {
-                    const double *S = g.sx + (size_t)((i - sh) * 2) * RA;
-                    double s = 0.0;
-#pragma unroll
-                    for (int m = 0; m < RA; m++) s += S[m] * P(0, wsx + sh + m);
-                    dxc0 = s;
-                }
-            }
             const int p = i;
-            const double d = g.cf[p], Ex = g.cf[nxy + p], Es = g.cf[3 * nxy + p];
-            double hx = 0.0, hs = 0.0;
-            if (g.bath) { hx = g.bath[p]; hs = g.bath[2 * nxy + p]; }
-            o[NZ - 1] = c[NZ - 1];
+            const double A2 = g.cf[p], B2 = g.cf[nxy + p], S5 = g.cf[6 * nxy + p];
+            const double *dxc = sDx + col * NZ;
+            if (k == 0) {
+                const int wz = cclamp(1 - A, 0, NZ - RA);
+                const double *Z1 = g.sz + (size_t)(1 * 2) * RA;
+                double dz = 0.0;
 #pragma unroll
-            for (int k = 1; k < NZ - 1; k++) {
+                for (int m = 0; m < RA; m++) dz += Z1[m] * P(wz + m, i);
+                const double hx = A2 * (1.0 / S5), sg = g.zs[1];
+                o = hx * dxc[1] + (S5 + hx * (A2 + sg * B2)) * dz;
+            } else {
+                const double A3 = g.cf[4 * nxy + p], B3 = g.cf[5 * nxy + p];
                 const int wz = cclamp(k - A, 0, NZ - RA);
                 const double *Z1 = g.sz + (size_t)(k * 2) * RA, *Z2 = Z1 + RA;
                 double dz = 0.0, dzz = 0.0, dxz = 0.0;
 #pragma unroll
                 for (int m = 0; m < RA; m++) {
-                    const double z1 = Z1[m];
-                    dz += z1 * c[wz + m];
-                    dzz += Z2[m] * c[wz + m];
-                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
+                    const double z1 = Z1[m], cv = P(wz + m, i);
+                    dz += z1 * cv;
+                    dzz += Z2[m] * cv;
+                    dxz += z1 * dxc[wz + m];
                 }
-                double s2, s3, s5;
-                sigma_coef2<double>(g.zs[k], d, hx, hs, Ex, Es, s2, s3, s5);
-                o[k] = axx[k] + 2.0 * s2 * dxz + (s2 * s2 + s5 * s5) * dzz + s3 * dz;
-            }
-            {
-                const int wz = cclamp(1 - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (size_t)(1 * 2) * RA;
-                double dz = 0.0;
-#pragma unroll
-                for (int m = 0; m < RA; m++) dz += Z1[m] * c[wz + m];
-                double s2, s3, s5;
-                sigma_coef2<double>(g.zs[1], d, hx, hs, Ex, Es, s2, s3, s5);
-                o[0] = hx * dx[1] + (s5 + hx * s2) * dz;
+                const double sg = g.zs[k];
+                const double s2 = A2 + sg * B2, s3 = A3 + sg * B3;
+                o = acc[r] + 2.0 * s2 * dxz + (s2 * s2 + S5 * S5) * dzz + s3 * dz;
             }
         }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int k = 0; k < NZ; k++) sP[(i - lox) * NZP + k] = o[k];
-    }
-    __syncthreads();
-    const int ie = cmin(i0 + TX, nxg);
-    for (int f = threadIdx.x; f < (ie - i0) * NZ; f += blockDim.x) {
-        const int ii = f / NZ, k = f - ii * NZ;
-        out[(size_t)i0 * NZ + f] = sP[(i0 + ii - lox) * NZP + k];
+        out[(size_t)i * NZ + k] = o;
     }
 }
 
@@ -332,6 +317,7 @@ __global__ __launch_bounds__(256) void k_matvec2(MvArgs g, const double *__restr
 struct PlaneArgs {
     int nxg, nyg, nxy, gx, gy, a, b, nonlinear;
     const double *sx, *sy, *h;
+    const double *bath;   // 3 planes: h_x, h_y, h_xx+h_yy
     double *E, *P, *Ex, *Exx, *Ey, *Eyy, *Px, *Py, *cf;
 };
 
@@ -362,10 +348,8 @@ __global__ __launch_bounds__(256) void k_surface_x(PlaneArgs g) {
         const size_t q = (size_t)j * nxg + i;
         g.Ex[q] = e1; g.Exx[q] = e2; g.Px[q] = p1;
         if (g.nyg == 1) {  // x-z problem: coefficient fields complete here
-            g.cf[q] = g.nonlinear ? E[i] + g.h[q] : g.h[q];
-            g.cf[g.nxy + q] = g.nonlinear ? e1 : 0.0;
-            g.cf[2 * g.nxy + q] = 0.0;
-            g.cf[3 * g.nxy + q] = g.nonlinear ? e2 : 0.0;
+            const double d = g.nonlinear ? E[i] + g.h[q] : g.h[q];
+            column_coefs(d, g.nonlinear ? e1 : 0.0, 0.0, g.nonlinear ? e2 : 0.0, g.bath[q], 0.0, g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
         }
     }
 }
@@ -398,14 +382,10 @@ __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
         }
         const size_t q = i + (size_t)nxg * j;
         g.Ey[q] = e1; g.Eyy[q] = e2; g.Py[q] = p1;
-        if (g.nonlinear) {
-            g.cf[q] = g.E[q] + g.h[q];
-            g.cf[g.nxy + q] = g.Ex[q];
-            g.cf[2 * g.nxy + q] = e1;
-            g.cf[3 * g.nxy + q] = g.Exx[q] + e2;
-        } else {
-            g.cf[q] = g.h[q]; g.cf[g.nxy + q] = 0.0; g.cf[2 * g.nxy + q] = 0.0; g.cf[3 * g.nxy + q] = 0.0;
-        }
+        if (g.nonlinear)
+            column_coefs(g.E[q] + g.h[q], g.Ex[q], e1, g.Exx[q] + e2, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
+        else
+            column_coefs(g.h[q], 0.0, 0.0, 0.0, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
     }
 }
 
@@ -417,7 +397,7 @@ __global__ void k_vertical_velocity(int nxy, int nzg, int rc, const double *__re
     const double *col = phi + (size_t)p * nzg + (nzg - rc);
     double s = 0.0;
     for (int m = 0; m < rc; m++) s += szTop[m] * col[m];
-    W[p] = (1.0 / cf[p]) * s;
+    W[p] = cf[6 * nxy + p] * s;   // dsigma/dz = 1/d
 }
 
 // Free-surface right-hand side (rhsFreeSurface3D.f90:13-28, rhsLinearFreeSurface3D.f90:11-12) fused with
@@ -487,12 +467,6 @@ __global__ void k_fill_rhs(int nxg, int nyg, int nzg, int gx, int gy, const doub
 __global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = a * x[q] + b * y[q];
-}
-// z = a*x + b*y + c*z  (extrapolations)
-__global__ void k_lin3(size_t n, double a, const double *__restrict__ x, double b, const double *__restrict__ y,
-                       double c, double *__restrict__ z) {
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-        z[q] = c * z[q] - b * y[q] + a * x[q];
 }
 // y = b - y
 __global__ void k_bminus(size_t n, const double *__restrict__ b, double *__restrict__ y) {
@@ -615,74 +589,6 @@ __device__ __forceinline__ void low_coefs(const LvArgs<T> &L, int p, int k, T &s
     const T d = L.col[p], hx = L.col[L.nxy + p], hy = L.col[2 * L.nxy + p], hs = L.col[3 * L.nxy + p];
     if (L.dim3) sigma_coef3<T>(L.zs[k], d, hx, hy, hs, T(0), T(0), T(0), s2, s3, s4, s5);
     else { sigma_coef2<T>(L.zs[k], d, hx, hs, T(0), T(0), s2, s3, s5); s4 = T(0); }
-}
-
-// r = b - L x on every row (thread per column).
-template <typename T, int NZ>
-__global__ __launch_bounds__(256) void k_mg_residual(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= L.nxy) return;
-    const int nxg = L.nxg, nyg = L.nyg, i = p % nxg, j = p / nxg;
-    const T *xc = x + (size_t)p * NZ;
-    const T *bc = b + (size_t)p * NZ;
-    T *rc = r + (size_t)p * NZ;
-    auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
-    const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
-    if (wx && wy) { for (int k = 0; k < NZ; k++) rc[k] = bc[k] - xc[k]; return; }
-    if (wx || wy) {
-        rc[0] = bc[0] - xc[0];
-        if (wx) {
-            const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
-            const T *S = L.sx + (ib * 2) * 3;
-            for (int k = 1; k < NZ; k++) rc[k] = bc[k] - ((S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j));
-        } else {
-            const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
-            const T *S = L.sy + (jb * 2) * 3;
-            for (int k = 1; k < NZ; k++) rc[k] = bc[k] - ((S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2));
-        }
-        return;
-    }
-    const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = L.dim3 ? cclamp(j - 1, 0, nyg - 3) : 0;
-    const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
-    const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
-    T dxl[NZ], dyl[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; k++) {
-        dxl[k] = (SX1[0] * X(k, wsx, j) + SX1[1] * X(k, wsx + 1, j)) + SX1[2] * X(k, wsx + 2, j);
-        dyl[k] = L.dim3 ? (SY1[0] * X(k, i, wsy) + SY1[1] * X(k, i, wsy + 1)) + SY1[2] * X(k, i, wsy + 2) : T(0);
-    }
-    {   // corner-corrected first pass on the ghost level
-        int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
-        if (sh) { const T *S = L.sx + ((i - sh) * 2) * 3; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
-        if (L.dim3) {
-            sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
-            if (sh) { const T *S = L.sy + ((j - sh) * 2) * 3; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
-        }
-    }
-    rc[NZ - 1] = bc[NZ - 1] - xc[NZ - 1];
-#pragma unroll
-    for (int k = 1; k < NZ - 1; k++) {
-        const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
-        const T dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
-        const T dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
-        const T dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
-        const T dyy = L.dim3 ? (SY2[0] * X(k, i, wsy) + SY2[1] * X(k, i, wsy + 1)) + SY2[2] * X(k, i, wsy + 2) : T(0);
-        const T dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
-        const T dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
-        T s2, s3, s4, s5;
-        low_coefs(L, p, k, s2, s3, s4, s5);
-        rc[k] = bc[k] - (dxx + dyy + s3 * dz + T(2) * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz);
-    }
-    {
-        const T *Z1 = L.sz + (1 * 2) * 3;
-        const T dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
-        T s2, s3, s4, s5;
-        low_coefs(L, p, 1, s2, s3, s4, s5);
-        const T hx = L.col[L.nxy + p], hy = L.dim3 ? L.col[2 * L.nxy + p] : T(0);
-        const T dx1 = (SX1[0] * X(1, wsx, j) + SX1[1] * X(1, wsx + 1, j)) + SX1[2] * X(1, wsx + 2, j);
-        const T dy1 = L.dim3 ? (SY1[0] * X(1, i, wsy) + SY1[1] * X(1, i, wsy + 1)) + SY1[2] * X(1, i, wsy + 2) : T(0);
-        rc[0] = bc[0] - ((s5 + hx * s2 + hy * s4) * dz + hx * dx1 + hy * dy1);
-    }
 }
 
 // Zebra z-line relaxation of the columns of one colour ((i+j) mod 2 == colour): exact tridiagonal
@@ -1071,52 +977,6 @@ __global__ __launch_bounds__(256) void k_mg_residual_t(LvArgs<T> L, const T *__r
         for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
             const int c = e / NZ, k = e - c * NZ;
             dst[e] = sX[((j0 + rr - loy) * rowstride + (i0 + c - lox)) * NZP + k];
-        }
-    }
-}
-
-// Wall ghost points: Neumann rows relaxed pointwise, identity rows set (GaussSeidelSpecialPoints.f90 role).
-template <typename T>
-__global__ void k_mg_ghost(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nxg = L.nxg, nyg = L.nyg, nzg = L.nzg;
-    int i, j;
-    if (L.dim3) {
-        const int nb = 2 * nxg + 2 * (nyg - 2);
-        if (t >= nb) return;
-        if (t < nxg) { i = t; j = 0; }
-        else if (t < 2 * nxg) { i = t - nxg; j = nyg - 1; }
-        else if (t < 2 * nxg + nyg - 2) { i = 0; j = 1 + t - 2 * nxg; }
-        else { i = nxg - 1; j = 1 + t - 2 * nxg - (nyg - 2); }
-    } else {
-        if (t >= 2) return;
-        i = t == 0 ? 0 : nxg - 1; j = 0;
-    }
-    const size_t base = ((size_t)j * nxg + i) * nzg;
-    const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
-    if (wx && wy) { for (int k = 0; k < nzg; k++) x[base + k] = b[base + k]; return; }
-    x[base] = b[base];
-    if (wx) {
-        const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
-        const T *S = L.sx + (ib * 2) * 3;
-        for (int k = 1; k < nzg; k++) {
-            T s = T(0), dg = T(1);
-            for (int m = 0; m < 3; m++) {
-                if (ws + m == i) dg = S[m];
-                else s += S[m] * x[((size_t)j * nxg + ws + m) * nzg + k];
-            }
-            x[base + k] = (b[base + k] - s) / dg;
-        }
-    } else {
-        const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
-        const T *S = L.sy + (jb * 2) * 3;
-        for (int k = 1; k < nzg; k++) {
-            T s = T(0), dg = T(1);
-            for (int m = 0; m < 3; m++) {
-                if (ws + m == j) dg = S[m];
-                else s += S[m] * x[((size_t)(ws + m) * nxg + i) * nzg + k];
-            }
-            x[base + k] = (b[base + k] - s) / dg;
         }
     }
 }

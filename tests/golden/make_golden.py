@@ -1,0 +1,90 @@
+"""Writes the golden vectors of tests/golden/*.npz from the reference's own Fortran routines
+(oracle/_ref, compiled from the reference sources by oracle/ref.mk; run in a container that holds the
+reference tree).  Each fixture holds the inputs (bathymetry, surface, potential) and the reference's
+outputs for them:
+
+  grid x, y, sigma levels            SetupCompDomain.f90:35-78
+  stencils sx, sy, sz                PreProcessDiffStencils.f90, BuildStencilsGridX/Y.f90
+  surface ghosts + derivatives       DifferentiationsFreeSurfacePlane.f90:12-32, UpdateGhostLayerX/Y.f90
+  dsigma (5 components)              DetermineTransformationConstantsArray.f90
+  A phi                              BuildLinearSystem.f90
+  W                                  VerticalFreeSurfaceVelocity.f90
+  rhsE, rhsP                         rhsFreeSurface3D.f90
+  restriction / prolongation         MGRestriction3DExplicit.f90, MGProlongation3DExplicit.f90
+
+Usage: python tests/golden/make_golden.py   (writes tests/golden/ref_<case>.npz)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oceanwave3d-fortran90_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle  # noqa: E402
+from ow3d import problems  # noqa: E402
+
+# name: (nx, ny, nz, alpha, shoal, gridz, L)
+CASES = {
+    "shoal17x17x6_o4": (17, 17, 6, 2, True, 1, 10.0),
+    "flat13x11x9_o6": (13, 11, 9, 3, False, 1, 6.0),
+    "shoal33x1x9_o6": (33, 1, 9, 3, True, 1, 10.0),
+    "shoal15x13x8_o8_even": (15, 13, 8, 4, True, 0, 8.0),
+}
+
+
+def inputs(nx, ny, nz, shoal, L, seed):
+    bath = (problems.gaussian_shoal if shoal else problems.flat_bottom)(nx, ny, L, L)
+    x, y = problems.coords(nx, ny, L, L)
+    X, Y = np.meshgrid(x, y)
+    rng = np.random.default_rng(seed)
+    E = 0.04 * np.sin(2 * np.pi * X / L + 0.4) * (np.cos(2 * np.pi * Y / L) if ny > 1 else 1.0) + 0.004 * rng.standard_normal(X.shape)
+    P = 0.2 * np.cos(2 * np.pi * X / L) + 0.02 * rng.standard_normal(X.shape)
+    nxg, nyg, nzg = nx + 2, (ny + 2 if ny > 1 else 1), nz + 1
+    phi = rng.standard_normal((nyg, nxg, nzg))
+    return bath, E, P, phi
+
+
+def main():
+    for seed, (name, (nx, ny, nz, alpha, shoal, gridz, L)) in enumerate(CASES.items()):
+        bath, E, P, phi = inputs(nx, ny, nz, shoal, L, seed)
+        R = oracle.RefTank(nx, ny, nz, L, L, alpha, gridz=gridz, bath=bath)
+        x, y, z = R.grid()
+        sx, sy, sz = R.stencils()
+        Er, Pr, d = R.surface(E, P)
+        Av = R.matvec(phi)
+        W = R.vertical_velocity(phi)
+        rE, rP = R.rhs(9.81, W)
+        out = dict(nx=nx, ny=ny, nz=nz, alpha=alpha, gridz=gridz, L=L, shoal=int(shoal),
+                   h=bath["h"], hx=bath["hx"], hy=bath["hy"], hxx=bath["hxx"], hyy=bath["hyy"],
+                   E_in=E, P_in=P, phi=phi, x=x, y=y, z=z, sx=sx, sy=sy, sz=sz, E=Er, P=Pr,
+                   dsigma=R.dsigma(), Aphi=Av, W=W, rhsE=rE, rhsP=rP, **{k: d[k] for k in d})
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **out)
+        print("wrote", name)
+    # multigrid transfers on a 17x17x9 hierarchy (levels as the oracle builds them)
+    from conftest import oracle_params
+    nx, ny, nz = 17, 17, 9
+    bath = problems.gaussian_shoal(nx, ny, 10.0, 10.0)
+    T = oracle.OracleTank(oracle_params(nx, ny, nz, lx=10.0, ly=10.0), bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+    out = dict(nx=nx, ny=ny, nz=nz, nlev=T.nlev, h=bath["h"], hx=bath["hx"], hy=bath["hy"], hxx=bath["hxx"], hyy=bath["hyy"])
+    rng = np.random.default_rng(99)
+    for fine in range(1, T.nlev):
+        f = T.level_grid(fine)
+        c = T.level_grid(fine - 1)
+        rf = rng.standard_normal((f["nyg"], f["nxg"], f["nzg"]))
+        ec = rng.standard_normal((c["nyg"], c["nxg"], c["nzg"]))
+        rc = oracle.ref_transfer(0, (f["nx"], f["ny"], f["nz"]), (f["x"], f["y"], f["z"]), (c["nx"], c["ny"], c["nz"]),
+                                 (c["x"], c["y"], c["z"]), rf, np.zeros((c["nyg"], c["nxg"], c["nzg"])))
+        ef = oracle.ref_transfer(1, (f["nx"], f["ny"], f["nz"]), (f["x"], f["y"], f["z"]), (c["nx"], c["ny"], c["nz"]),
+                                 (c["x"], c["y"], c["z"]), np.zeros((f["nyg"], f["nxg"], f["nzg"])), ec)
+        out.update({f"rf{fine}": rf, f"rc{fine}": rc, f"ec{fine}": ec, f"ef{fine}": ef})
+    np.savez_compressed(os.path.join(HERE, "ref_transfer17x17x9.npz"), **out)
+    print("wrote transfer17x17x9")
+
+
+if __name__ == "__main__":
+    main()
