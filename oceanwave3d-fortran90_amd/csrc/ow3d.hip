@@ -88,6 +88,9 @@ struct ow3d_tank {
     std::vector<Grid> levels;
     std::vector<std::unique_ptr<MgLevel<double>>> mg64;
     std::vector<std::unique_ptr<MgLevel<float>>> mg32;
+    int bot_top = -1;                 // finest level handled by the one-workgroup bottom kernel (-1: none)
+    BottomArgs<double> bot64{};
+    BottomArgs<float> bot32{};
     // statistics
     long long it_total = 0, it_solves = 0;
     int it_min = -1, it_max = 0;
@@ -111,14 +114,15 @@ namespace {
 template <int NZ, int A, int TY>
 constexpr size_t mv3_lds() {
     constexpr int R = A + 1, RA = 2 * A + 1;
-    return ((size_t)(16 + 2 * R) * (TY + 2 * R) * NZ + 2 * 16 * TY * NZ + (16 + 2) * 2 * RA + (TY + 2) * 2 * RA + NZ * 2 * RA + NZ) * 8;
+    return ((size_t)(16 + 2 * R) * (TY + 2 * R) * NZ + 2 * 16 * TY * NZ + 7 * 16 * TY + (16 + 2) * 2 * RA + (TY + 2) * 2 * RA +
+            NZ * 2 * RA + NZ) * 8;
 }
 template <int NZ, int A>
 int launch_mv3(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
     constexpr int TY = mv3_lds<NZ, A, 8>() <= 65536 ? 8 : 4;
     static_assert(mv3_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
-    dim3 grid((t->g.nxg + 15) / 16, (t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), grid, dim3(256), 0, s, t->mv, phi, y);
+    const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
+    hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), dim3(nb), dim3(256), 0, s, t->mv, phi, y);
     return 0;
 }
 template <int NZ, int A>
@@ -181,8 +185,7 @@ void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T 
             hipLaunchKernelGGL((k_mg_zebra_t<T, NZ>), grid, dim3(128), 0, s, L, colour, x, b);
         } else {
             const int half = (nxi + 1) / 2;
-            dim3 grid((half + 63) / 64, nyi);
-            hipLaunchKernelGGL((k_mg_zebra<T, NZ>), grid, dim3(64), 0, s, L, colour, x, b);
+            hipLaunchKernelGGL((k_mg_zebra<T, NZ>), dim3((half * nyi + 255) / 256), dim3(256), 0, s, L, colour, x, b);
         }
     }
 }
@@ -357,6 +360,21 @@ RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const double *yf
 }
 
 template <typename T>
+void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, BottomArgs<T> &A) {
+    A = BottomArgs<T>{};
+    if (t->bot_top < 0) return;
+    A.nlev = t->bot_top + 1;
+    A.nu1 = t->cfg.nu1; A.nu2 = t->cfg.nu2;
+    A.n0 = (int)mg[0]->g.n;
+    A.minv = mg[0]->minv.p;
+    for (int k = 0; k <= t->bot_top; k++) {
+        MgLevel<T> &L = *mg[k];
+        A.L[k] = L.args; A.x[k] = L.x.p; A.b[k] = L.b.p; A.r[k] = L.r.p;
+        if (k >= 1) A.G[k] = geo(L.g, mg[k - 1]->g, L.xf.p, L.yf.p, L.zf.p);
+    }
+}
+
+template <typename T>
 void ghost_pass(MgLevel<T> &L, hipStream_t s) {
     const Grid &g = L.g;
     const int nb = (g.ny > 1 ? 2 * g.nxg + 2 * (g.nyg - 2) : 2) * g.nzg;
@@ -375,9 +393,17 @@ int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, hipStream_t s) {
     return 0;
 }
 
+template <typename T> const BottomArgs<T> &bottom_args(const ow3d_tank *t);
+template <> const BottomArgs<double> &bottom_args<double>(const ow3d_tank *t) { return t->bot64; }
+template <> const BottomArgs<float> &bottom_args<float>(const ow3d_tank *t) { return t->bot32; }
+
 template <typename T>
 int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, hipStream_t s) {
     MgLevel<T> &L = *mg[k];
+    if (k == t->bot_top) {
+        hipLaunchKernelGGL((k_mg_bottom<T>), dim3(1), dim3(512), 0, s, bottom_args<T>(t));
+        return 0;
+    }
     if (k == 0) {
         hipLaunchKernelGGL((k_mg_coarse<T>), dim3(1), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
         return 0;
@@ -386,8 +412,7 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, hi
     if (smooth(t, L, t->cfg.nu1, s)) return -1;
     if (mg_launch<T>(L.args, 0, 0, L.x.p, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p);
-    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p);
-    hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, C.g.n, T(0), C.x.p);
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
     if (vcycle(t, mg, k - 1, s)) return -1;
     hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks(L.g.n)), dim3(256), 0, s, G, (const T *)C.x.p, L.x.p);
     if (smooth(t, L, t->cfg.nu2, s)) return -1;
@@ -456,10 +481,44 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
 int red_blocks(size_t n) { return (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + 1023) / 1024)); }
 
 // out[0..k-1] (device hv) = V_i . w ; copied to host vals.
+// Krylov kernel dispatch on the number of basis vectors (1 .. MAXK-1).
+#define OW3D_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+static_assert(MAXK == 32, "Krylov dispatch covers 1..31 vectors");
+
+int launch_mdot(int k, bool norm, int nb, size_t n, const double *V, const double *w, double *part, hipStream_t s) {
+    switch (k) {
+#define OW3D_C(K) case K: if (norm) hipLaunchKernelGGL((k_mdot<K, 1, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part); \
+                          else hipLaunchKernelGGL((k_mdot<K, 0, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part); return 0;
+        OW3D_K_CASES(OW3D_C)
+#undef OW3D_C
+    }
+    g_err = "Krylov dimension out of range";
+    return -1;
+}
+int launch_gs_update_dot(int k, int nb, size_t n, const double *V, const double *h, double *v, double *part, hipStream_t s) {
+    switch (k) {
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update_dot<K, MAXK>), dim3(nb), dim3(256), 0, s, n, V, h, v, part); return 0;
+        OW3D_K_CASES(OW3D_C)
+#undef OW3D_C
+    }
+    g_err = "Krylov dimension out of range";
+    return -1;
+}
+int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, double sign, double scale, double *v, hipStream_t s) {
+    switch (k) {
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update<K>), dim3(nb), dim3(256), 0, s, n, V, h, sign, scale, v); return 0;
+        OW3D_K_CASES(OW3D_C)
+#undef OW3D_C
+    }
+    g_err = "Krylov dimension out of range";
+    return -1;
+}
+
 int mdot(ow3d_tank *t, int k, const double *V, const double *w, double *vals, hipStream_t s) {
     const size_t n = t->g.n;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_mdot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, V, w, t->part.p, 0);
+    if (launch_mdot(k, false, nb, n, V, w, t->part.p, s)) return -1;
     hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)t->part.p, t->hv.p);
     HIPCHK(hipMemcpyAsync(vals, t->hv.p, k * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -509,7 +568,7 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
         if (precond(t, w, vn, s)) return -1;
         // classical Gram-Schmidt with the DGKS re-orthogonalisation test (second pass only when the
         // projection removed more than half of the squared norm)
-        hipLaunchKernelGGL((k_mdot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)vn, t->part.p, 1);
+        if (launch_mdot(k, true, nb, n, V, vn, t->part.p, s)) return -1;
         hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
         HIPCHK(hipMemcpyAsync(h.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -521,7 +580,7 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
             hn = std::sqrt(nn);
             for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i];
         } else {
-            hipLaunchKernelGGL((k_gs_update_dot<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)t->hv.p, vn, t->part.p);
+            if (launch_gs_update_dot(k, nb, n, V, t->hv.p, vn, t->part.p, s)) return -1;
             hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
             HIPCHK(hipMemcpyAsync(h2.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -532,8 +591,7 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
         }
         H[(size_t)k * m + (k - 1)] = hn;
         // vn = (vn - V h) / hn  with the h of the last pass (in t->hv)
-        hipLaunchKernelGGL((k_gs_update<MAXK>), dim3(nb), dim3(256), 0, s, n, k, (const double *)V, (const double *)t->hv.p, -1.0,
-                           hn > 0.0 ? 1.0 / hn : 1.0, vn);
+        if (launch_gs_update(k, nb, n, V, t->hv.p, -1.0, hn > 0.0 ? 1.0 / hn : 1.0, vn, s)) return -1;
         for (int i = 0; i < k - 1; i++) {
             double a = H[(size_t)i * m + (k - 1)], bb = H[(size_t)(i + 1) * m + (k - 1)];
             H[(size_t)i * m + (k - 1)] = cs[i] * a + sn[i] * bb;
@@ -562,7 +620,7 @@ int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hip
     }
     if (kk > 0) {
         HIPCHK(hipMemcpyAsync(t->hv.p, y.data(), kk * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL((k_gs_update<MAXK>), dim3(nb), dim3(256), 0, s, n, kk, (const double *)V, (const double *)t->hv.p, 1.0, 1.0, x);
+        if (launch_gs_update(kk, nb, n, V, t->hv.p, 1.0, 1.0, x, s)) return -1;
     }
     HIPCHK(hipStreamSynchronize(s));
     *iters = matvecs;
@@ -719,6 +777,14 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
             if (L.nzg < 3 || L.nzg > 17) { g_err = "multigrid level depth outside 3..17"; return -1; }
         int rc = c.mg_precision == 0 ? mg_build<double>(t.get(), t->mg64) : mg_build<float>(t.get(), t->mg32);
         if (rc) return -1;
+        // levels small enough for the one-workgroup bottom kernel (OW3D_MG_BOTTOM: point limit, 0 = off)
+        long lim = 50000;
+        if (const char *e = std::getenv("OW3D_MG_BOTTOM")) lim = std::atol(e);
+        t->bot_top = -1;
+        for (int k = 0; k < t->nlev && k < MG_BOTTOM_MAX; k++)
+            if ((long)t->levels[k].n <= lim) t->bot_top = k;
+        if (c.mg_precision == 0) fill_bottom<double>(t.get(), t->mg64, t->bot64);
+        else fill_bottom<float>(t.get(), t->mg32, t->bot32);
         if (t->levels[0].n > 4096) { g_err = "coarsest multigrid level too large for the dense solve"; return -1; }
     }
     HIPCHK(hipDeviceSynchronize());

@@ -68,6 +68,37 @@ __device__ __forceinline__ void column_coefs(double d, double Ex, double Ey, dou
     cf[6 * nxy + p] = id;
 }
 
+// Copies a window of rows x cols sigma-columns (NZ values each) of a field in HBM into LDS: window row r
+// starts at src + r * pitch; LDS holds element (r, c, k) at (r * RS + c) * NZP + k.  All loads of a thread
+// are issued before its first LDS store, so a block waits for HBM once, not once per row.
+template <typename T, int NZ, int NZP, int MR, int MC, int RS, int NT>
+__device__ __forceinline__ void stage_window(T *lds, const T *__restrict__ src, size_t pitch, int rows, int cols) {
+    constexpr int RL = MC * NZ, TOT = MR * RL, PER = (TOT + NT - 1) / NT;
+    T v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = e / RL, c = e - r * RL;
+        v[q] = (e < TOT && r < rows && c < cols * NZ) ? src[(size_t)r * pitch + c] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = e / RL, c = e - r * RL;
+        if (e < TOT && r < rows && c < cols * NZ) {
+            const int cc = c / NZ, k = c - cc * NZ;
+            lds[(r * RS + cc) * NZP + k] = v[q];
+        }
+    }
+}
+
+// Block index -> tile index such that each of the 8 XCDs (which receive blocks round-robin) works on a
+// contiguous range of tiles, so halo columns shared by neighbouring tiles are re-read from the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb / 8, rem = nb % 8, xcd = b % 8, slot = b / 8;
+    return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
+}
+
 // 3D, one thread per grid point: block = 16 x TY columns x NZ levels, NT threads.  The tile is
 // stored k-fastest in LDS exactly as in HBM (row segments are straight copies), and threads are mapped
 // k-fastest too, so every LDS access of a wave and every global load / store is contiguous.
@@ -84,15 +115,30 @@ __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restr
     __shared__ double sSY[(TY + 2) * 2 * RA];
     __shared__ double sSZ[NZ * 2 * RA];
     __shared__ double sZS[NZ];
+    __shared__ double sCF[7 * NC];
     const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int gxt = (nxg + TX - 1) / TX;
+    const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int i0 = (tile % gxt) * TX, j0 = (tile / gxt) * TY;
     const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
     const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
-    for (int jj = 0; jj < wy; jj++) {
-        const double *src = phi + ((size_t)(loy + jj) * nxg + lox) * NZ;
-        double *dst = sP + jj * WX * NZ;
-        for (int e = threadIdx.x; e < wx * NZ; e += NT) dst[e] = src[e];
+    {
+        // coefficient planes of the tile's columns, loaded together with the phi window
+        constexpr int CPT = (7 * NC + NT - 1) / NT;
+        double cv[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int pl = e / NC, c = e - pl * NC, ii = i0 + c % TX, jj = j0 + c / TX;
+            cv[q] = (e < 7 * NC && ii < nxg && jj < nyg) ? g.cf[(size_t)pl * nxy + ii + (size_t)nxg * jj] : 0.0;
+        }
+        stage_window<double, NZ, NZ, WY, WX, WX, NT>(sP, phi + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wy, wx);
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            if (e < 7 * NC) sCF[e] = cv[q];
+        }
     }
     for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
         const int r = e / (2 * RA), ii = i0 - 1 + r;
@@ -180,9 +226,8 @@ __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restr
 #pragma unroll
             for (int m = 0; m < RA; m++) o += S[m] * P(k, i, ws + m);
         } else {
-            const int p = i + nxg * j;
-            const double A2 = g.cf[p], B2 = g.cf[nxy + p], A4 = g.cf[2 * nxy + p], B4 = g.cf[3 * nxy + p];
-            const double S5 = g.cf[6 * nxy + p];
+            const double A2 = sCF[col], B2 = sCF[NC + col], A4 = sCF[2 * NC + col], B4 = sCF[3 * NC + col];
+            const double S5 = sCF[6 * NC + col];
             const double *dxc = sDx + col * NZ, *dyc = sDy + col * NZ;
             if (k == 0) {
                 const int wz = cclamp(1 - A, 0, NZ - RA);
@@ -194,7 +239,7 @@ __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restr
                 const double sg = sZS[1], d = 1.0 / S5, hx = A2 * d, hy = A4 * d;
                 o = (S5 + hx * (A2 + sg * B2) + hy * (A4 + sg * B4)) * dz + hx * dxc[1] + hy * dyc[1];
             } else {
-                const double A3 = g.cf[4 * nxy + p], B3 = g.cf[5 * nxy + p];
+                const double A3 = sCF[4 * NC + col], B3 = sCF[5 * NC + col];
                 const int wz = cclamp(k - A, 0, NZ - RA);
                 const double *Z1 = sSZ + (k * 2) * RA, *Z2 = Z1 + RA;
                 double dz = 0.0, dzz = 0.0, dxz = 0.0, dyz = 0.0;
@@ -225,7 +270,7 @@ __global__ __launch_bounds__(256) void k_matvec2p(MvArgs g, const double *__rest
     const int i0 = blockIdx.x * TX;
     const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox);
-    for (int e = threadIdx.x; e < wx * NZ; e += NT) sP[e] = phi[(size_t)lox * NZ + e];
+    stage_window<double, NZ, NZ, 1, WX, WX, NT>(sP, phi + (size_t)lox * NZ, 0, 1, wx);
     __syncthreads();
     auto P = [&](int k, int ii) -> double { return sP[(ii - lox) * NZ + k]; };
     double acc[PER];
@@ -485,75 +530,84 @@ __global__ void k_fill(size_t n, T v, T *__restrict__ y) {
 }
 
 // Block reduction of MAXK accumulators into part[blockIdx.x*MAXK + i].
-template <int MAXK>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[MAXK], int kk, double *part) {
-    __shared__ double red[MAXK][4];
+// Block sum of NA per-thread accumulators, stored at part[blockIdx.x * PSTRIDE + i] (fixed order).
+template <int NA, int PSTRIDE>
+__device__ __forceinline__ void block_reduce_store(const double (&acc)[NA], double *part) {
+    __shared__ double red[NA][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < MAXK; i++) {
+    for (int i = 0; i < NA; i++) {
         double v = acc[i];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
         if (lane == 0) red[i][wv] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kk) {
+    if (threadIdx.x < NA) {
         const int i = threadIdx.x;
-        part[(size_t)blockIdx.x * MAXK + i] = ((red[i][0] + red[i][1]) + red[i][2]) + red[i][3];
+        part[(size_t)blockIdx.x * PSTRIDE + i] = ((red[i][0] + red[i][1]) + red[i][2]) + red[i][3];
     }
 }
 
-// part[b][i] = sum over block b's range of V_i . w, i < k  (V_i = V + i*n); with_norm adds w . w at index k.
-template <int MAXK>
-__global__ __launch_bounds__(256) void k_mdot(size_t n, int k, const double *__restrict__ V, const double *__restrict__ w, double *part,
-                                               int with_norm) {
-    double acc[MAXK];
+// The Krylov kernels below take the number of basis vectors K as a template parameter: every load of a
+// grid point is issued before the first use (no predicated loads, no waits between them).
+
+// part[b][i] = sum over block b's range of V_i . w for i < K (V_i = V + i*n); NORM adds w . w at index K.
+template <int K, int NORM, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_mdot(size_t n, const double *__restrict__ V, const double *__restrict__ w,
+                                               double *__restrict__ part) {
+    double acc[K + NORM];
 #pragma unroll
-    for (int i = 0; i < MAXK; i++) acc[i] = 0.0;
+    for (int i = 0; i < K + NORM; i++) acc[i] = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double v[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) v[i] = V[(size_t)i * n + q];
         const double wq = w[q];
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) {
-            if (i < k) acc[i] += V[(size_t)i * n + q] * wq;
-            else if (i == k && with_norm) acc[i] += wq * wq;
-        }
+        for (int i = 0; i < K; i++) acc[i] += v[i] * wq;
+        if (NORM) acc[K + NORM - 1] += wq * wq;
     }
-    block_reduce_store<MAXK>(acc, k + (with_norm ? 1 : 0), part);
+    block_reduce_store<K + NORM, PSTRIDE>(acc, part);
 }
 
-// v <- v - sum_i h_i V_i; part gets the dots of the new v with V_i (i < k) and its squared norm (index k).
-template <int MAXK>
-__global__ __launch_bounds__(256) void k_gs_update_dot(size_t n, int k, const double *__restrict__ V, const double *__restrict__ h,
-                                                        double *__restrict__ v, double *part) {
-    double acc[MAXK];
-    double hh[MAXK];
+// v <- v - sum_i h_i V_i; part gets the dots of the new v with V_i (i < K) and its squared norm (index K).
+template <int K, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_gs_update_dot(size_t n, const double *__restrict__ V, const double *__restrict__ h,
+                                                        double *__restrict__ v, double *__restrict__ part) {
+    double acc[K + 1], hh[K];
 #pragma unroll
-    for (int i = 0; i < MAXK; i++) { acc[i] = 0.0; hh[i] = (i < k) ? h[i] : 0.0; }
+    for (int i = 0; i < K; i++) { acc[i] = 0.0; hh[i] = h[i]; }
+    acc[K] = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double vv[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) vv[i] = V[(size_t)i * n + q];
         double x = v[q];
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) if (i < k) x -= hh[i] * V[(size_t)i * n + q];
+        for (int i = 0; i < K; i++) x -= hh[i] * vv[i];
         v[q] = x;
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) {
-            if (i < k) acc[i] += V[(size_t)i * n + q] * x;
-            else if (i == k) acc[i] += x * x;
-        }
+        for (int i = 0; i < K; i++) acc[i] += vv[i] * x;
+        acc[K] += x * x;
     }
-    block_reduce_store<MAXK>(acc, k + 1, part);
+    block_reduce_store<K + 1, PSTRIDE>(acc, part);
 }
 
-// v <- (v - sum_i h_i V_i) * scale, plus the solution update form x <- x + sum_i h_i V_i (sign = +1).
-template <int MAXK>
-__global__ __launch_bounds__(256) void k_gs_update(size_t n, int k, const double *__restrict__ V, const double *__restrict__ h,
+// v <- (v + sign * sum_i h_i V_i) * scale  (sign -1: orthogonalisation; +1, scale 1: solution update).
+template <int K>
+__global__ __launch_bounds__(256) void k_gs_update(size_t n, const double *__restrict__ V, const double *__restrict__ h,
                                                     double sign, double scale, double *__restrict__ v) {
-    double hh[MAXK];
+    double hh[K];
 #pragma unroll
-    for (int i = 0; i < MAXK; i++) hh[i] = (i < k) ? h[i] : 0.0;
+    for (int i = 0; i < K; i++) hh[i] = sign * h[i];
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double vv[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) vv[i] = V[(size_t)i * n + q];
         double x = v[q];
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) if (i < k) x += sign * hh[i] * V[(size_t)i * n + q];
+        for (int i = 0; i < K; i++) x += hh[i] * vv[i];
         v[q] = x * scale;
     }
 }
@@ -591,20 +645,14 @@ __device__ __forceinline__ void low_coefs(const LvArgs<T> &L, int p, int k, T &s
     else { sigma_coef2<T>(L.zs[k], d, hx, hs, T(0), T(0), s2, s3, s5); s4 = T(0); }
 }
 
-// Zebra z-line relaxation of the columns of one colour ((i+j) mod 2 == colour): exact tridiagonal
-// solve in sigma with the neighbouring columns frozen.
-template <typename T, int NZ>
-__global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+// Zebra z-line relaxation of column (i, j): exact tridiagonal solve in sigma of the level operator's
+// rows with the neighbouring columns frozen (read through X(k, i, j)); the new column goes to out[0..NZ).
+// Couplings to same-colour columns (|di| or |dj| = 2, from the corner-shifted first pass on the sigma
+// ghost level) are left out, so that all columns of one colour can be relaxed at once.
+template <typename T, int NZ, class XA>
+__device__ __forceinline__ void zebra_column(const LvArgs<T> &L, int i, int j, const XA &X, const T *bc, T *out) {
     const int nxg = L.nxg, nyg = L.nyg;
-    const int nxi = nxg - 2 * L.gx;
-    const int j = L.dim3 ? (int)blockIdx.y + L.gy : 0;
-    if (j >= nyg - L.gy) return;
-    const int ifirst = L.gx + (((L.gx + j) & 1) != colour ? 1 : 0);
-    const int i = ifirst + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= L.gx + nxi) return;
     const int p = i + nxg * j;
-    auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
-    const T *bc = b + (size_t)p * NZ;
     const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = L.dim3 ? cclamp(j - 1, 0, nyg - 3) : 0;
     const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
     const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
@@ -655,7 +703,7 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
                 const T w = T(2) * s2 * Z1[mz] * S1[m];
                 const int ii = wsx + sh + m;
                 if (ii == i) a[mz] += w;
-                else if (ii - i != 2 && i - ii != 2) off += w * X(kk, ii, j);  // same-colour column: not coupled in the sweep
+                else if (ii - i != 2 && i - ii != 2) off += w * X(kk, ii, j);
             }
         }
         if (L.dim3) {
@@ -670,7 +718,7 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
                     const T w = T(2) * s4 * Z1[mz] * S1[m];
                     const int jj = wsy + sh + m;
                     if (jj == j) a[mz] += w;
-                    else if (jj - j != 2 && j - jj != 2) off += w * X(kk, i, jj);  // same-colour column: not coupled in the sweep
+                    else if (jj - j != 2 && j - jj != 2) off += w * X(kk, i, jj);
                 }
             }
         }
@@ -688,155 +736,83 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
         di[k] -= f * up[k - 1];
         rh[k] -= f * rh[k - 1];
     }
-    T *xc = x + (size_t)p * NZ;
-    T xn[NZ];
-    xn[NZ - 1] = rh[NZ - 1] / di[NZ - 1];
+    T xn = rh[NZ - 1] / di[NZ - 1];
+    out[NZ - 1] = xn;
 #pragma unroll
-    for (int k = NZ - 2; k >= 0; k--) xn[k] = (rh[k] - up[k] * xn[k + 1]) / di[k];
-#pragma unroll
-    for (int k = 0; k < NZ; k++) xc[k] = xn[k];
+    for (int k = NZ - 2; k >= 0; k--) { xn = (rh[k] - up[k] * xn) / di[k]; out[k] = xn; }
 }
 
-// LDS-tiled zebra z-line relaxation: a block stages a 32 x 8 tile of columns (plus a one-column
-// halo) and the tile's right-hand sides, 128 threads each solve the sweep-colour column of one
-// (even, odd) pair, and the whole tile is written back coalesced (columns of the other colour
-// are rewritten unchanged; nothing else modifies them during the sweep).
-template <typename T, int NZ>
-__global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
-    constexpr int TX = 32, TY = 8, WX = TX + 2, WY = TY + 2, NZP = NZ | 1;
-    __shared__ T sX[WX * WY * NZP];
-    __shared__ T sB[TX * TY * NZP];
+// Residual o = b - L x of column (i, j) of a level (all row kinds: interior, bottom ghost, surface,
+// wall ghost columns, corners), x read through X(k, i, j).
+template <typename T, int NZ, class XA>
+__device__ __forceinline__ void residual_column(const LvArgs<T> &L, int i, int j, const XA &X, const T *bc, T *o) {
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
-    const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
-    const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
-    const int lox = i0 - 1, loy = d3 ? j0 - 1 : 0;
-    const int wcols = cmin(WX, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
-    for (int r = 0; r < wrows; r++) {
-        const T *src = x + ((size_t)(loy + r) * nxg + lox) * NZ;
-        for (int e = threadIdx.x; e < wcols * NZ; e += blockDim.x) {
-            const int c = e / NZ, k = e - c * NZ;
-            sX[(r * WX + c) * NZP + k] = src[e];
+    const int p = i + nxg * j;
+    const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
+    if (wx && wy) {
+#pragma unroll
+        for (int k = 0; k < NZ; k++) o[k] = bc[k] - X(k, i, j);
+    } else if (wx || wy) {
+        o[0] = bc[0] - X(0, i, j);
+        if (wx) {
+            const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
+            const T *S = L.sx + (ib * 2) * 3;
+#pragma unroll
+            for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j));
+        } else {
+            const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
+            const T *S = L.sy + (jb * 2) * 3;
+#pragma unroll
+            for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2));
         }
-    }
-    for (int r = 0; r < je - j0; r++) {
-        const T *src = b + ((size_t)(j0 + r) * nxg + i0) * NZ;
-        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
-            const int c = e / NZ, k = e - c * NZ;
-            sB[(r * TX + c) * NZP + k] = src[e];
-        }
-    }
-    __syncthreads();
-    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
-    const int ty = threadIdx.x / 16, pr = threadIdx.x % 16;
-    const int j = j0 + ty;
-    int i = i0 + 2 * pr;
-    if (((i + j) & 1) != colour) i++;
-    if (j < je && i < ie) {
-        const int p = i + nxg * j;
-        const T *bc = sB + ((j - j0) * TX + (i - i0)) * NZP;
+    } else {
         const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
         const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
         const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
-        int shx = 0; if (wsx == 0) shx = 1; if (wsx + 2 == nxg - 1) shx = -1;
-        int shy = 0;
-        if (d3) { if (wsy == 0) shy = 1; if (wsy + 2 == nyg - 1) shy = -1; }
-        const T *SXc = L.sx + ((i - shx) * 2) * 3;
-        const T *SYc = L.sy + ((j - shy) * 2) * 3;
-        T lo[NZ], di[NZ], up[NZ], rh[NZ];
-        T e2 = T(0);
+        T xc[NZ], dxl[NZ], dyl[NZ];
 #pragma unroll
-        for (int k = 0; k < NZ; k++) { lo[k] = T(0); di[k] = T(0); up[k] = T(0); }
-        {
-            const T *Sz = L.sz + (1 * 2) * 3;
-            T s2, s3, s4, s5;
-            low_coefs(L, p, 1, s2, s3, s4, s5);
-            const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
-            const T cz = s5 + hx * s2 + hy * s4;
-            T a0 = cz * Sz[0], a1 = cz * Sz[1], a2 = cz * Sz[2], off = T(0);
-            for (int m = 0; m < 3; m++) {
-                if (wsx + m == i) a1 += hx * SX1[m]; else off += hx * SX1[m] * X(1, wsx + m, j);
-            }
-            if (d3)
-                for (int m = 0; m < 3; m++) {
-                    if (wsy + m == j) a1 += hy * SY1[m]; else off += hy * SY1[m] * X(1, i, wsy + m);
-                }
-            di[0] = a0; up[0] = a1; e2 = a2;
-            rh[0] = bc[0] - off;
+        for (int k = 0; k < NZ; k++) {
+            xc[k] = X(k, i, j);
+            dxl[k] = (SX1[0] * X(k, wsx, j) + SX1[1] * X(k, wsx + 1, j)) + SX1[2] * X(k, wsx + 2, j);
+            dyl[k] = d3 ? (SY1[0] * X(k, i, wsy) + SY1[1] * X(k, i, wsy + 1)) + SY1[2] * X(k, i, wsy + 2) : T(0);
         }
+        {
+            int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
+            if (sh) { const T *S = L.sx + ((i - sh) * 2) * 3; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
+            if (d3) {
+                sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
+                if (sh) { const T *S = L.sy + ((j - sh) * 2) * 3; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
+            }
+        }
+        o[NZ - 1] = bc[NZ - 1] - xc[NZ - 1];
 #pragma unroll
         for (int k = 1; k < NZ - 1; k++) {
             const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
+            const T dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
+            const T dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
+            const T dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
+            const T dyy = d3 ? (SY2[0] * X(k, i, wsy) + SY2[1] * X(k, i, wsy + 1)) + SY2[2] * X(k, i, wsy + 2) : T(0);
+            const T dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
+            const T dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
             T s2, s3, s4, s5;
             low_coefs(L, p, k, s2, s3, s4, s5);
-            const T czz = s2 * s2 + s4 * s4 + s5 * s5;
-            T a[3];
-            for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
-            T off = T(0);
-            for (int m = 0; m < 3; m++) {
-                if (wsx + m == i) a[1] += SX2[m]; else off += SX2[m] * X(k, wsx + m, j);
-            }
-            for (int mz = 0; mz < 3; mz++) {
-                const int kk = k - 1 + mz;
-                const int sh = (kk == 0) ? shx : 0;
-                const T *S1 = (kk == 0) ? SXc : SX1;
-                for (int m = 0; m < 3; m++) {
-                    const T w = T(2) * s2 * Z1[mz] * S1[m];
-                    const int ii = wsx + sh + m;
-                    if (ii == i) a[mz] += w;
-                    else if (ii - i != 2 && i - ii != 2) off += w * X(kk, ii, j);
-                }
-            }
-            if (d3) {
-                for (int m = 0; m < 3; m++) {
-                    if (wsy + m == j) a[1] += SY2[m]; else off += SY2[m] * X(k, i, wsy + m);
-                }
-                for (int mz = 0; mz < 3; mz++) {
-                    const int kk = k - 1 + mz;
-                    const int sh = (kk == 0) ? shy : 0;
-                    const T *S1 = (kk == 0) ? SYc : SY1;
-                    for (int m = 0; m < 3; m++) {
-                        const T w = T(2) * s4 * Z1[mz] * S1[m];
-                        const int jj = wsy + sh + m;
-                        if (jj == j) a[mz] += w;
-                        else if (jj - j != 2 && j - jj != 2) off += w * X(kk, i, jj);
-                    }
-                }
-            }
-            lo[k] = a[0]; di[k] = a[1]; up[k] = a[2];
-            rh[k] = bc[k] - off;
+            o[k] = bc[k] - (dxx + dyy + s3 * dz + T(2) * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz);
         }
-        di[NZ - 1] = T(1); rh[NZ - 1] = bc[NZ - 1];
-        if (e2 != T(0)) {
-            const T f = e2 / up[1];
-            di[0] -= f * lo[1]; up[0] -= f * di[1]; rh[0] -= f * rh[1];
-        }
-#pragma unroll
-        for (int k = 1; k < NZ; k++) {
-            const T f = lo[k] / di[k - 1];
-            di[k] -= f * up[k - 1];
-            rh[k] -= f * rh[k - 1];
-        }
-        T *own = sX + ((j - loy) * WX + (i - lox)) * NZP;
-        T xn = rh[NZ - 1] / di[NZ - 1];
-        own[NZ - 1] = xn;
-#pragma unroll
-        for (int k = NZ - 2; k >= 0; k--) { xn = (rh[k] - up[k] * xn) / di[k]; own[k] = xn; }
-    }
-    __syncthreads();
-    for (int r = 0; r < je - j0; r++) {
-        T *dst = x + ((size_t)(j0 + r) * nxg + i0) * NZ;
-        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
-            const int c = e / NZ, k = e - c * NZ;
-            dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
+        {
+            const T *Z1 = L.sz + (1 * 2) * 3;
+            const T dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
+            T s2, s3, s4, s5;
+            low_coefs(L, p, 1, s2, s3, s4, s5);
+            const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
+            o[0] = bc[0] - ((s5 + hx * s2 + hy * s4) * dz + hx * dxl[1] + hy * dyl[1]);
         }
     }
 }
 
-// Wall ghost points, one thread per (ghost column, level): Neumann rows relaxed pointwise, identity
-// rows set to the right-hand side (the role of GaussSeidelSpecialPoints.f90).
+// Wall ghost point t of a level (t enumerates (ghost column, sigma level) pairs): Neumann rows relaxed
+// pointwise, identity rows set to the right-hand side (the role of GaussSeidelSpecialPoints.f90).
 template <typename T>
-__global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ghost_point(const LvArgs<T> &L, int t, T *__restrict__ x, const T *__restrict__ b) {
     const int nxg = L.nxg, nyg = L.nyg, nzg = L.nzg;
     const int col = t / nzg, k = t - col * nzg;
     int i, j;
@@ -872,6 +848,71 @@ __global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict
     }
     x[q] = (b[q] - s) / dg;
 }
+template <typename T>
+__device__ __forceinline__ int ghost_count(const LvArgs<T> &L) {
+    return (L.dim3 ? 2 * L.nxg + 2 * (L.nyg - 2) : 2) * L.nzg;
+}
+
+// Column index c of the sweep colour -> (i, j); false past the last column of that colour.
+template <typename T>
+__device__ __forceinline__ bool colour_column(const LvArgs<T> &L, int colour, int c, int &i, int &j) {
+    const int nxi = L.nxg - 2 * L.gx, half = (nxi + 1) / 2;
+    const int jr = c / half;
+    j = L.dim3 ? jr + L.gy : 0;
+    if (j >= (L.dim3 ? L.nyg - L.gy : 1)) return false;
+    i = L.gx + (((L.gx + j) & 1) != colour ? 1 : 0) + 2 * (c - jr * half);
+    return i < L.gx + nxi;
+}
+
+// Zebra sweep over the columns of one colour, one thread per column, x read and written in HBM.
+template <typename T, int NZ>
+__global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    int i, j;
+    if (!colour_column(L, colour, (int)(blockIdx.x * blockDim.x + threadIdx.x), i, j)) return;
+    const int nxg = L.nxg;
+    auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
+    const size_t p = (size_t)i + (size_t)nxg * j;
+    zebra_column<T, NZ>(L, i, j, X, b + p * NZ, x + p * NZ);
+}
+
+// LDS-tiled zebra z-line relaxation: a block stages a 32 x 8 tile of columns (plus a one-column
+// halo) and the tile's right-hand sides, 128 threads each solve the sweep-colour column of one
+// (even, odd) pair, and the whole tile is written back coalesced (columns of the other colour
+// are rewritten unchanged; nothing else modifies them during the sweep).
+template <typename T, int NZ>
+__global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    constexpr int TX = 32, TY = 8, WX = TX + 2, WY = TY + 2, NZP = NZ | 1;
+    __shared__ T sX[WX * WY * NZP];
+    __shared__ T sB[TX * TY * NZP];
+    const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
+    const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
+    const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
+    const int lox = i0 - 1, loy = d3 ? j0 - 1 : 0;
+    const int wcols = cmin(WX, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
+    stage_window<T, NZ, NZP, WY, WX, WX, 128>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    stage_window<T, NZ, NZP, TY, TX, TX, 128>(sB, b + ((size_t)j0 * nxg + i0) * NZ, (size_t)nxg * NZ, je - j0, ie - i0);
+    __syncthreads();
+    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
+    const int ty = threadIdx.x / 16, pr = threadIdx.x % 16;
+    const int j = j0 + ty;
+    int i = i0 + 2 * pr;
+    if (((i + j) & 1) != colour) i++;
+    if (j < je && i < ie)
+        zebra_column<T, NZ>(L, i, j, X, sB + ((j - j0) * TX + (i - i0)) * NZP, sX + ((j - loy) * WX + (i - lox)) * NZP);
+    __syncthreads();
+    for (int r = 0; r < je - j0; r++) {
+        T *dst = x + ((size_t)(j0 + r) * nxg + i0) * NZ;
+        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+            const int c = e / NZ, k = e - c * NZ;
+            dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
+    ghost_point(L, (int)(blockIdx.x * blockDim.x + threadIdx.x), x, b);
+}
 
 // LDS-tiled residual r = b - L x (16 x 16 columns per block, halo 2 on domain ends as the matvec).
 template <typename T, int NZ>
@@ -879,91 +920,23 @@ __global__ __launch_bounds__(256) void k_mg_residual_t(LvArgs<T> L, const T *__r
     constexpr int TX = 16, TY = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1;
     __shared__ T sX[WX * WY * NZP];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
-    const int tyd = d3 ? TY : 1;
     const int i0 = d3 ? (int)blockIdx.x * TX : (int)blockIdx.x * (TX * TY), j0 = d3 ? (int)blockIdx.y * TY : 0;
     const int tw = d3 ? TX : TX * TY;        // columns per tile row
     const int lox = cmax(cmin(i0 - R, nxg - 4), 0);
     const int loy = d3 ? cmax(cmin(j0 - R, nyg - 4), 0) : 0;
-    const int wxmax = d3 ? WX : WX * WY;     // 2D: the tile is one long row
     const int wcols = cmin(tw + 2 * R, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
-    (void)wxmax;
     const int rowstride = d3 ? WX : (TX * TY + 2 * R);
-    for (int rr = 0; rr < wrows; rr++) {
-        const T *src = x + ((size_t)(loy + rr) * nxg + lox) * NZ;
-        for (int e = threadIdx.x; e < wcols * NZ; e += blockDim.x) {
-            const int c = e / NZ, k = e - c * NZ;
-            sX[(rr * rowstride + c) * NZP + k] = src[e];
-        }
-    }
+    if (d3)
+        stage_window<T, NZ, NZP, WY, WX, WX, 256>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    else
+        stage_window<T, NZ, NZP, 1, TX * TY + 2 * R, TX * TY + 2 * R, 256>(sX, x + (size_t)lox * NZ, 0, 1, wcols);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * rowstride + (ii - lox)) * NZP + k]; };
     const int tx = d3 ? (int)threadIdx.x % TX : (int)threadIdx.x, ty = d3 ? (int)threadIdx.x / TX : 0;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = i < nxg && j < nyg && (d3 || ty == 0);
     T o[NZ];
-    if (active) {
-        const int p = i + nxg * j;
-        const T *bc = b + (size_t)p * NZ;
-        const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
-        if (wx && wy) {
-#pragma unroll
-            for (int k = 0; k < NZ; k++) o[k] = bc[k] - X(k, i, j);
-        } else if (wx || wy) {
-            o[0] = bc[0] - X(0, i, j);
-            if (wx) {
-                const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
-                const T *S = L.sx + (ib * 2) * 3;
-#pragma unroll
-                for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j));
-            } else {
-                const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
-                const T *S = L.sy + (jb * 2) * 3;
-#pragma unroll
-                for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2));
-            }
-        } else {
-            const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
-            const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
-            const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
-            T xc[NZ], dxl[NZ], dyl[NZ];
-#pragma unroll
-            for (int k = 0; k < NZ; k++) {
-                xc[k] = X(k, i, j);
-                dxl[k] = (SX1[0] * X(k, wsx, j) + SX1[1] * X(k, wsx + 1, j)) + SX1[2] * X(k, wsx + 2, j);
-                dyl[k] = d3 ? (SY1[0] * X(k, i, wsy) + SY1[1] * X(k, i, wsy + 1)) + SY1[2] * X(k, i, wsy + 2) : T(0);
-            }
-            {
-                int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
-                if (sh) { const T *S = L.sx + ((i - sh) * 2) * 3; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
-                if (d3) {
-                    sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
-                    if (sh) { const T *S = L.sy + ((j - sh) * 2) * 3; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
-                }
-            }
-            o[NZ - 1] = bc[NZ - 1] - xc[NZ - 1];
-#pragma unroll
-            for (int k = 1; k < NZ - 1; k++) {
-                const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
-                const T dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
-                const T dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
-                const T dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
-                const T dyy = d3 ? (SY2[0] * X(k, i, wsy) + SY2[1] * X(k, i, wsy + 1)) + SY2[2] * X(k, i, wsy + 2) : T(0);
-                const T dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
-                const T dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
-                T s2, s3, s4, s5;
-                low_coefs(L, p, k, s2, s3, s4, s5);
-                o[k] = bc[k] - (dxx + dyy + s3 * dz + T(2) * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz);
-            }
-            {
-                const T *Z1 = L.sz + (1 * 2) * 3;
-                const T dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
-                T s2, s3, s4, s5;
-                low_coefs(L, p, 1, s2, s3, s4, s5);
-                const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
-                o[0] = bc[0] - ((s5 + hx * s2 + hy * s4) * dz + hx * dxl[1] + hy * dyl[1]);
-            }
-        }
-    }
+    if (active) residual_column<T, NZ>(L, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, o);
     __syncthreads();
     if (active) {
         T *dst = sX + ((j - loy) * rowstride + (i - lox)) * NZP;
@@ -989,11 +962,12 @@ struct RestrictGeo {
     int xc, yc, zc;
     const double *xf, *yf, *zf;                      // fine coordinates
 };
+// Coarse point q; also clears the coarse iterate xz (the coarse V cycle starts from zero).
 template <typename T>
-__global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__restrict__ rc) {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void restrict_point(const RestrictGeo &G, size_t q, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
     const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
     if (q >= nc) return;
+    xz[q] = T(0);
     const int kc = (int)(q % G.nzgc);
     const size_t col = q / G.nzgc;
     const int ic = (int)(col % G.nxgc), jc = (int)(col / G.nxgc);
@@ -1027,12 +1001,15 @@ __global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__rest
     } else v = Yv(kf, ifn, jfn);
     rc[q] = v;
 }
+template <typename T>
+__global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
+    restrict_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, rf, rc, xz);
+}
 
 // Prolongation (MGProlongation3DExplicit.f90) of the coarse correction, added to x on interior
 // columns below the free surface.
 template <typename T>
-__global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__restrict__ xf) {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T *__restrict__ xf) {
     const size_t nf = (size_t)G.nxgf * G.nygf * G.nzgf;
     if (q >= nf) return;
     const int k = (int)(q % G.nzgf);
@@ -1072,6 +1049,10 @@ __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__r
     } else e = EY(k, i, j);
     xf[q] = xf[q] + e;
 }
+template <typename T>
+__global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__restrict__ xf) {
+    prolong_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, ec, xf);
+}
 
 // Coarsest level: x = Minv b (one block).
 template <typename T>
@@ -1080,6 +1061,108 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const T *__restrict__ 
         T s = T(0);
         for (int c = 0; c < n; c++) s += Minv[(size_t)r * n + c] * b[c];
         x[r] = s;
+    }
+}
+
+
+// The bottom of the V cycle in one workgroup: levels 0 .. nlev-1 of a hierarchy (0 = coarsest) whose
+// finest member `top` = nlev-1 arrives with its right-hand side set and x = 0.  Runs the same sequence
+// as the multi-kernel V cycle (ghost pass, nu1 x (red, black, ghost), residual, restriction, ..., dense
+// coarse solve, ..., prolongation, ghost pass, nu2 x (red, black, ghost)), with the level vectors in HBM /
+// L2 and a workgroup barrier between the steps.  Small levels cost one barrier per step instead of one
+// kernel launch.
+constexpr int MG_BOTTOM_MAX = 10;
+template <typename T>
+struct BottomArgs {
+    int nlev, nu1, nu2, n0;
+    const T *minv;
+    LvArgs<T> L[MG_BOTTOM_MAX];
+    T *x[MG_BOTTOM_MAX];
+    T *b[MG_BOTTOM_MAX];
+    T *r[MG_BOTTOM_MAX];
+    RestrictGeo G[MG_BOTTOM_MAX];   // G[l]: level l (fine) -> level l-1 (coarse), l >= 1
+};
+
+template <typename T, int NZ>
+__device__ void bottom_sweep(const LvArgs<T> &L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    const int nxi = L.nxg - 2 * L.gx, half = (nxi + 1) / 2, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
+    const int nxg = L.nxg;
+    auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
+    for (int c = threadIdx.x; c < half * nyi; c += blockDim.x) {
+        int i, j;
+        if (!colour_column(L, colour, c, i, j)) continue;
+        const size_t p = (size_t)i + (size_t)nxg * j;
+        zebra_column<T, NZ>(L, i, j, X, b + p * NZ, x + p * NZ);
+    }
+}
+
+template <typename T, int NZ>
+__device__ void bottom_residual(const LvArgs<T> &L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
+    const int nxg = L.nxg;
+    auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
+    for (int c = threadIdx.x; c < L.nxy; c += blockDim.x) {
+        const int i = c % nxg, j = c / nxg;
+        T o[NZ];
+        residual_column<T, NZ>(L, i, j, X, b + (size_t)c * NZ, o);
+#pragma unroll
+        for (int k = 0; k < NZ; k++) r[(size_t)c * NZ + k] = o[k];
+    }
+}
+
+template <typename T>
+__device__ void bottom_dispatch(int what, const LvArgs<T> &L, int colour, T *x, const T *b, T *r) {
+    switch (L.nzg) {
+#define OW3D_BZ(N) case N: if (what == 0) bottom_residual<T, N>(L, x, b, r); else bottom_sweep<T, N>(L, colour, x, b); break;
+        OW3D_BZ(3) OW3D_BZ(4) OW3D_BZ(5) OW3D_BZ(6) OW3D_BZ(7) OW3D_BZ(8) OW3D_BZ(9) OW3D_BZ(10)
+        OW3D_BZ(11) OW3D_BZ(12) OW3D_BZ(13) OW3D_BZ(14) OW3D_BZ(15) OW3D_BZ(16) OW3D_BZ(17)
+#undef OW3D_BZ
+    }
+}
+
+template <typename T>
+__device__ void bottom_ghost(const LvArgs<T> &L, T *x, const T *b) {
+    const int nb = ghost_count(L);
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) ghost_point(L, t, x, b);
+}
+
+template <typename T>
+__device__ void bottom_smooth(const LvArgs<T> &L, int nu, T *x, const T *b) {
+    bottom_ghost(L, x, b);
+    __syncthreads();
+    for (int it = 0; it < nu; it++) {
+        bottom_dispatch<T>(1, L, 0, x, b, nullptr);
+        __syncthreads();
+        bottom_dispatch<T>(1, L, 1, x, b, nullptr);
+        __syncthreads();
+        bottom_ghost(L, x, b);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_mg_bottom(BottomArgs<T> A) {
+    const int top = A.nlev - 1;
+    for (int l = top; l >= 1; l--) {
+        bottom_smooth(A.L[l], A.nu1, A.x[l], A.b[l]);
+        bottom_dispatch<T>(0, A.L[l], 0, A.x[l], A.b[l], A.r[l]);
+        __syncthreads();
+        const RestrictGeo &G = A.G[l];
+        const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
+        for (size_t q = threadIdx.x; q < nc; q += blockDim.x) restrict_point(G, q, A.r[l], A.b[l - 1], A.x[l - 1]);
+        __syncthreads();
+    }
+    for (int rr = threadIdx.x; rr < A.n0; rr += blockDim.x) {
+        T s = T(0);
+        for (int c = 0; c < A.n0; c++) s += A.minv[(size_t)rr * A.n0 + c] * A.b[0][c];
+        A.x[0][rr] = s;
+    }
+    __syncthreads();
+    for (int l = 1; l <= top; l++) {
+        const RestrictGeo &G = A.G[l];
+        const size_t nf = (size_t)G.nxgf * G.nygf * G.nzgf;
+        for (size_t q = threadIdx.x; q < nf; q += blockDim.x) prolong_point(G, q, A.x[l - 1], A.x[l]);
+        __syncthreads();
+        bottom_smooth(A.L[l], A.nu2, A.x[l], A.b[l]);
     }
 }
 
