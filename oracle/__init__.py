@@ -308,6 +308,7 @@ def ref_lib():
         R.ref_lowmatrix_get.argtypes = [D, IP, IP]
         R.ref_gs.argtypes = [I, I, IP, IP, D, D, D, I, IP]
         R.ref_transfer.argtypes = [I, I, I, I, D, D, D, I, I, I, D, D, D, D, D]
+        R.ref_store_ep.argtypes = [D, D, I]
         _ref = R
     return _ref
 
@@ -369,6 +370,17 @@ class RefTank:
         rE = np.zeros(self.shape2); rP = np.zeros(self.shape2)
         ref_lib().ref_rhs(g, _p(W), _p(rE), _p(rP))
         return rE, rP
+
+    def store_ep(self, directory, E, P, nr=0):
+        """StoreData's EP_<nr>.bin for E, P, written into `directory`; returns the file path."""
+        E = np.ascontiguousarray(E, dtype=np.float64); P = np.ascontiguousarray(P, dtype=np.float64)
+        cwd = os.getcwd()
+        try:
+            os.chdir(directory)
+            ref_lib().ref_store_ep(_p(E), _p(P), int(nr))
+        finally:
+            os.chdir(cwd)
+        return os.path.join(directory, "EP_%05d.bin" % nr)
 
 
 def ref_lowmatrix(nx, ny, nz, x, y, z, bath):

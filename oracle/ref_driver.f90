@@ -290,4 +290,15 @@ CONTAINS
     DEALLOCATE(F%x, F%y, F%z, C%x, C%y, C%z)
   END SUBROUTINE ref_transfer
 
+
+  ! StoreData (IO/StoreData.f90) on the grid of the last ref_setup: writes EP_<nr>.bin in the working
+  ! directory, E and P over the ghost-extended plane.
+  SUBROUTINE ref_store_ep(E, P, nr) BIND(C, name="ref_store_ep")
+    REAL(C_DOUBLE), INTENT(IN) :: E(*), P(*)
+    INTEGER(C_INT), VALUE :: nr
+    INTEGER :: nxg, nyg
+    nxg = FineGrid%Nx + 2*GhostGridX; nyg = FineGrid%Ny + 2*GhostGridY
+    CALL StoreData(nxg, nyg, E(1:nxg*nyg), P(1:nxg*nyg), FineGrid, INT(nr), 1)
+  END SUBROUTINE ref_store_ep
+
 END MODULE ow3d_ref_driver
