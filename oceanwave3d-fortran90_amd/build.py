@@ -8,22 +8,40 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "ow3d", "libow3d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed-FP32 instructions (v_pk_*_f32) in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+SOURCES = ["ow3d.hip", "ow3d_matvec.hip", "ow3d_mg.hip", "ow3d_mg64.hip", "ow3d_mg32.hip"]
 
 
 def build(verbose=False):
-    src = os.path.join(CSRC, "ow3d.hip")
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", src]
+    """Compiles the translation units in parallel, then links libow3d.so (written atomically)."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for src, obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError("hipcc failed on " + ", ".join(failed))
+    cmd = [HIPCC] + FLAGS + ["-shared", "-o", OUT + ".tmp"] + [o for _, o, _ in procs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libow3d.so")
+        raise RuntimeError("hipcc failed linking libow3d.so")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

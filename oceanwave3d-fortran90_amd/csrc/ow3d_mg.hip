@@ -1,0 +1,63 @@
+// Geometric multigrid preconditioner (multigrid/*.f90 of the reference, with a zebra z-line smoother):
+// hierarchy set-up and the preconditioner entry with its hipGraph replay.  The per-precision work is in
+// ow3d_mg64.hip / ow3d_mg32.hip.
+#include "ow3d_internal.hpp"
+
+namespace ow3d_host {
+
+// Multigrid hierarchy of the tank (MGPreProcess.f90:6-460 with coarsening strategy 0), level operators in
+// the precision the configuration asks for, and the levels handed to the one-workgroup bottom kernel.
+int mg_setup(ow3d_tank *t) {
+    const ow3d_config &c = t->cfg;
+    if (!mg_levels(t->g, c.mg_maxgrids, t->levels, err())) return -1;
+    t->nlev = (int)t->levels.size();
+    for (auto &L : t->levels)
+        if (L.nzg < 3 || L.nzg > 14) { err() = "multigrid level depth outside 3..14"; return -1; }
+    if (t->levels[0].n > 4096) { err() = "coarsest multigrid level too large for the dense solve"; return -1; }
+    int rc = c.mg_precision == 0 ? mg_build64(t) : mg_build32(t);
+    if (rc) return -1;
+    // levels small enough for the one-workgroup bottom kernel (OW3D_MG_BOTTOM: point limit, 0 = off)
+    long lim = 50000;
+    if (const char *e = std::getenv("OW3D_MG_BOTTOM")) lim = std::atol(e);
+    if (const char *e = std::getenv("OW3D_MG_FUSED_RB")) t->fused_rb = e[0] != '0';
+    t->bot_top = -1;
+    for (int k = 0; k < t->nlev && k < MG_BOTTOM_MAX; k++)
+        if ((long)t->levels[k].n <= lim) t->bot_top = k;
+    if (c.mg_precision == 0) mg_fill_bottom64(t);
+    else mg_fill_bottom32(t);
+    return 0;
+}
+
+// u = M^-1 r (device vectors, FP64 in / out).  When r is the handle's residual buffer the V cycle is
+// replayed from a hipGraph captured on first use (one graph per handle; fixed buffers).
+int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
+    const size_t n = t->g.n;
+    if (t->cfg.precond != 3) {
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, u);
+        return 0;
+    }
+    if (t->use_graph && r == t->w.p && s == t->s) {
+        if (!t->pg_exec) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, s) : precond_body32(t, r, s));
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return -1; }
+            HIPCHK(e);
+            HIPCHK(hipGraphInstantiate(&t->pg_exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+        }
+        HIPCHK(hipGraphLaunch(t->pg_exec, s));
+    } else {
+        if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, s) : precond_body32(t, r, s))) return -1;
+    }
+    const int top = t->nlev - 1;
+    if (t->cfg.mg_precision == 0)
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg64[top]->x.p, u);
+    else
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg32[top]->x.p, u);
+    return 0;
+}
+
+
+}  // namespace ow3d_host

@@ -1,0 +1,286 @@
+// Multigrid preconditioner internals shared by the per-precision translation units ow3d_mg64.hip and
+// ow3d_mg32.hip (level set-up, kernel dispatch over the column depth, the V cycle).
+#pragma once
+#include "ow3d_internal.hpp"
+
+namespace ow3d_host {
+namespace mg {
+
+template <typename T, int NZ>
+void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, hipStream_t s) {
+    if (what == 0) {
+        if (L.dim3) {
+            dim3 grid((L.nxg + 15) / 16, (L.nyg + 15) / 16);
+            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
+        } else {
+            dim3 grid((L.nxg + 255) / 256, 1);
+            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
+        }
+    } else if (what == 2) {
+        const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
+        dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
+        hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ>), grid, dim3(128), 0, s, L, x, b);
+    } else {
+        const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
+        constexpr size_t lds = (size_t)(34 * 10 + 32 * 8) * (NZ | 1) * sizeof(T);
+        if constexpr (lds <= 65536) {
+            dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
+            hipLaunchKernelGGL((k_mg_zebra_t<T, NZ>), grid, dim3(128), 0, s, L, colour, x, b);
+        } else {
+            const int half = (nxi + 1) / 2;
+            hipLaunchKernelGGL((k_mg_zebra<T, NZ>), dim3((half * nyi + 255) / 256), dim3(256), 0, s, L, colour, x, b);
+        }
+    }
+}
+
+template <typename T>
+int mg_launch(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, hipStream_t s) {
+    switch (L.nzg) {
+#define OW3D_NZ(N) case N: mg_launch_nz<T, N>(L, what, colour, x, b, r, s); return 0;
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multigrid set-up and V cycle (MultiGridSolver.f90:5-192 structure with the zebra smoother)
+// ---------------------------------------------------------------------------------------------
+inline void dense_inverse(const std::vector<double> &A, int n, std::vector<double> &Ainv) {
+    std::vector<double> M(A);
+    std::vector<int> piv(n);
+    for (int c = 0; c < n; c++) {
+        int pr = c; double best = std::fabs(M[(size_t)c * n + c]);
+        for (int r = c + 1; r < n; r++) if (std::fabs(M[(size_t)r * n + c]) > best) { best = std::fabs(M[(size_t)r * n + c]); pr = r; }
+        piv[c] = pr;
+        if (pr != c) for (int q = 0; q < n; q++) std::swap(M[(size_t)c * n + q], M[(size_t)pr * n + q]);
+        for (int r = c + 1; r < n; r++) {
+            double f = M[(size_t)r * n + c] / M[(size_t)c * n + c];
+            M[(size_t)r * n + c] = f;
+            for (int q = c + 1; q < n; q++) M[(size_t)r * n + q] -= f * M[(size_t)c * n + q];
+        }
+    }
+    Ainv.assign((size_t)n * n, 0.0);
+    std::vector<double> b(n);
+    for (int col = 0; col < n; col++) {
+        std::fill(b.begin(), b.end(), 0.0); b[col] = 1.0;
+        for (int c = 0; c < n; c++) if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+        for (int r = 0; r < n; r++) { double s = b[r]; for (int q = 0; q < r; q++) s -= M[(size_t)r * n + q] * b[q]; b[r] = s; }
+        for (int r = n - 1; r >= 0; r--) { double s = b[r]; for (int q = r + 1; q < n; q++) s -= M[(size_t)r * n + q] * b[q]; b[r] = s / M[(size_t)r * n + r]; }
+        for (int r = 0; r < n; r++) Ainv[(size_t)r * n + col] = b[r];
+    }
+}
+
+// Host evaluation of the level operator applied to a unit vector (coarsest level assembly).
+// Same row semantics as k_mg_residual_t, in double.
+inline void low_apply_host(const Grid &g, const std::vector<double> &x, std::vector<double> &y) {
+    const int nxg = g.nxg, nyg = g.nyg, nzg = g.nzg, nxy = g.nxy;
+    const bool d3 = g.ny > 1;
+    auto X = [&](int k, int i, int j) { return x[((size_t)j * nxg + i) * nzg + k]; };
+    auto coefs = [&](int p, int k, double &s2, double &s3, double &s4, double &s5) {
+        double zs = k == 0 ? 0.0 : g.z[k], d = g.h[p], hx = g.hx[p], hy = d3 ? g.hy[p] : 0.0;
+        double hs = g.hxx[p] + (d3 ? g.hyy[p] : 0.0);
+        double s1 = zs;
+        s2 = ((1.0 - s1) * hx - s1 * 0.0) / d;
+        if (d3) s3 = -(2.0 * (1.0 - s1) * (hx * hx + hy * hy) / (d * d)) + 2.0 * (2.0 * s1 - 1.0) * (0.0 * hx + 0.0 * hy) / (d * d) + (1.0 - s1) * hs / d + 2.0 * s1 * 0.0 / (d * d) - s1 * 0.0 / d;
+        else s3 = -(2.0 * (1.0 - s1) * (hx * hx) / (d * d)) + 2.0 * (2.0 * s1 - 1.0) * (0.0 * hx) / (d * d) + (1.0 - s1) * hs / d + 2.0 * s1 * 0.0 / (d * d) - s1 * 0.0 / d;
+        s4 = d3 ? ((1.0 - s1) * hy - s1 * 0.0) / d : 0.0;
+        s5 = 1.0 / d;
+    };
+    y.assign(g.n, 0.0);
+    for (int p = 0; p < nxy; p++) {
+        int i = p % nxg, j = p / nxg;
+        double *yc = &y[(size_t)p * nzg];
+        bool wx = g.gx && (i == 0 || i == nxg - 1), wy = g.gy && (j == 0 || j == nyg - 1);
+        if (wx && wy) { for (int k = 0; k < nzg; k++) yc[k] = X(k, i, j); continue; }
+        if (wx || wy) {
+            yc[0] = X(0, i, j);
+            if (wx) {
+                int ib = i == 0 ? 1 : nxg - 2, ws = clampi(ib - 1, 0, nxg - 3);
+                const double *S = &g.sx[(size_t)(ib * 2) * 3];
+                for (int k = 1; k < nzg; k++) yc[k] = (S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j);
+            } else {
+                int jb = j == 0 ? 1 : nyg - 2, ws = clampi(jb - 1, 0, nyg - 3);
+                const double *S = &g.sy[(size_t)(jb * 2) * 3];
+                for (int k = 1; k < nzg; k++) yc[k] = (S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2);
+            }
+            continue;
+        }
+        int wsx = clampi(i - 1, 0, nxg - 3), wsy = d3 ? clampi(j - 1, 0, nyg - 3) : 0;
+        const double *SX1 = &g.sx[(size_t)(i * 2) * 3], *SX2 = SX1 + 3;
+        const double *SY1 = d3 ? &g.sy[(size_t)(j * 2) * 3] : nullptr, *SY2 = d3 ? SY1 + 3 : nullptr;
+        std::vector<double> dxl(nzg), dyl(nzg, 0.0);
+        for (int k = 0; k < nzg; k++) {
+            dxl[k] = (SX1[0] * X(k, wsx, j) + SX1[1] * X(k, wsx + 1, j)) + SX1[2] * X(k, wsx + 2, j);
+            if (d3) dyl[k] = (SY1[0] * X(k, i, wsy) + SY1[1] * X(k, i, wsy + 1)) + SY1[2] * X(k, i, wsy + 2);
+        }
+        int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
+        if (sh) { const double *S = &g.sx[(size_t)((i - sh) * 2) * 3]; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
+        if (d3) {
+            sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
+            if (sh) { const double *S = &g.sy[(size_t)((j - sh) * 2) * 3]; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
+        }
+        const double *xc = &x[(size_t)p * nzg];
+        yc[nzg - 1] = xc[nzg - 1];
+        for (int k = 1; k < nzg - 1; k++) {
+            const double *Z1 = &g.sz[(size_t)(k * 2) * 3], *Z2 = Z1 + 3;
+            double dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
+            double dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
+            double dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
+            double dyy = d3 ? (SY2[0] * X(k, i, wsy) + SY2[1] * X(k, i, wsy + 1)) + SY2[2] * X(k, i, wsy + 2) : 0.0;
+            double dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
+            double dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
+            double s2, s3, s4, s5; coefs(p, k, s2, s3, s4, s5);
+            yc[k] = dxx + dyy + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz;
+        }
+        const double *Z1 = &g.sz[(size_t)(1 * 2) * 3];
+        double dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
+        double s2, s3, s4, s5; coefs(p, 1, s2, s3, s4, s5);
+        double hx = g.hx[p], hy = d3 ? g.hy[p] : 0.0;
+        double dx1 = (SX1[0] * X(1, wsx, j) + SX1[1] * X(1, wsx + 1, j)) + SX1[2] * X(1, wsx + 2, j);
+        double dy1 = d3 ? (SY1[0] * X(1, i, wsy) + SY1[1] * X(1, i, wsy + 1)) + SY1[2] * X(1, i, wsy + 2) : 0.0;
+        yc[0] = (s5 + hx * s2 + hy * s4) * dz + hx * dx1 + hy * dy1;
+    }
+}
+
+template <typename T>
+int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
+    hipStream_t s = t->s;
+    mg.clear();
+    for (int k = 0; k < t->nlev; k++) {
+        auto L = std::make_unique<MgLevel<T>>();
+        L->g = t->levels[k];
+        const Grid &g = L->g;
+        auto up = [&](DevBuf<T> &d, const std::vector<double> &src) -> hipError_t {
+            std::vector<T> tmp(src.begin(), src.end());
+            hipError_t e = d.alloc(tmp.size());
+            if (e != hipSuccess) return e;
+            e = hipMemcpy(d.p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice);
+            return e;
+        };
+        std::vector<double> sy = g.sy;
+        if (g.ny == 1) sy.assign(6, 0.0);
+        std::vector<double> zs(g.z); zs[0] = 0.0;
+        std::vector<double> col((size_t)4 * g.nxy);
+        for (int p = 0; p < g.nxy; p++) {
+            col[p] = g.h[p]; col[g.nxy + p] = g.hx[p]; col[2 * g.nxy + p] = g.ny > 1 ? g.hy[p] : 0.0;
+            col[3 * g.nxy + p] = g.hxx[p] + (g.ny > 1 ? g.hyy[p] : 0.0);
+        }
+        HIPCHK(up(L->sx, g.sx)); HIPCHK(up(L->sy, sy)); HIPCHK(up(L->sz, g.sz)); HIPCHK(up(L->zs, zs)); HIPCHK(up(L->col, col));
+        HIPCHK(L->x.alloc(g.n)); HIPCHK(L->b.alloc(g.n)); HIPCHK(L->r.alloc(g.n));
+        HIPCHK(L->xf.alloc(g.nxg)); HIPCHK(L->yf.alloc(g.nyg)); HIPCHK(L->zf.alloc(g.nzg));
+        HIPCHK(hipMemcpy(L->xf.p, g.x.data(), g.nxg * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L->yf.p, g.y.data(), g.nyg * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L->zf.p, g.z.data(), g.nzg * 8, hipMemcpyHostToDevice));
+        if (k == 0) {
+            int n = (int)g.n;
+            std::vector<double> A((size_t)n * n), e(n, 0.0), colv;
+            for (int c = 0; c < n; c++) {
+                e[c] = 1.0; low_apply_host(g, e, colv); e[c] = 0.0;
+                for (int r = 0; r < n; r++) A[(size_t)r * n + c] = colv[r];
+            }
+            std::vector<double> Ainv;
+            dense_inverse(A, n, Ainv);
+            HIPCHK(up(L->minv, Ainv));
+        }
+        L->args.nxg = g.nxg; L->args.nyg = g.nyg; L->args.nzg = g.nzg; L->args.nxy = g.nxy;
+        L->args.gx = g.gx; L->args.gy = g.gy; L->args.dim3 = g.ny > 1;
+        L->args.sx = L->sx.p; L->args.sy = L->sy.p; L->args.sz = L->sz.p; L->args.zs = L->zs.p; L->args.col = L->col.p;
+        mg.push_back(std::move(L));
+    }
+    (void)s;
+    return 0;
+}
+
+inline RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const double *yf, const double *zf) {
+    RestrictGeo G{};
+    G.nxgf = f.nxg; G.nygf = f.nyg; G.nzgf = f.nzg; G.nxf = f.nx; G.nyf = f.ny; G.nzf = f.nz; G.gx = f.gx; G.gy = f.gy;
+    G.nxgc = c.nxg; G.nygc = c.nyg; G.nzgc = c.nzg; G.nxc = c.nx; G.nyc = c.ny; G.nzc = c.nz;
+    G.xc = c.xc; G.yc = c.yc; G.zc = c.zc;
+    G.xf = xf; G.yf = yf; G.zf = zf;
+    return G;
+}
+
+template <typename T>
+void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, BottomArgs<T> &A) {
+    A = BottomArgs<T>{};
+    if (t->bot_top < 0) return;
+    A.nlev = t->bot_top + 1;
+    A.nu1 = t->cfg.nu1; A.nu2 = t->cfg.nu2;
+    A.n0 = (int)mg[0]->g.n;
+    A.minv = mg[0]->minv.p;
+    for (int k = 0; k <= t->bot_top; k++) {
+        MgLevel<T> &L = *mg[k];
+        A.L[k] = L.args; A.x[k] = L.x.p; A.b[k] = L.b.p; A.r[k] = L.r.p;
+        if (k >= 1) A.G[k] = geo(L.g, mg[k - 1]->g, L.xf.p, L.yf.p, L.zf.p);
+    }
+}
+
+template <typename T>
+void ghost_pass(MgLevel<T> &L, hipStream_t s) {
+    const Grid &g = L.g;
+    const int nb = (g.ny > 1 ? 2 * g.nxg + 2 * (g.nyg - 2) : 2) * g.nzg;
+    hipLaunchKernelGGL((k_mg_ghost_p<T>), dim3((nb + 255) / 256), dim3(256), 0, s, L.args, L.x.p, (const T *)L.b.p);
+}
+
+template <typename T>
+int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, hipStream_t s) {
+    ghost_pass(L, s);
+    for (int it = 0; it < nu; it++) {
+        if (t->fused_rb) {
+            if (mg_launch<T>(L.args, 2, 0, L.x.p, L.b.p, nullptr, s)) return -1;
+        } else {
+            if (mg_launch<T>(L.args, 1, 0, L.x.p, L.b.p, nullptr, s)) return -1;
+            if (mg_launch<T>(L.args, 1, 1, L.x.p, L.b.p, nullptr, s)) return -1;
+        }
+        ghost_pass(L, s);
+    }
+    (void)t;
+    return 0;
+}
+
+template <typename T> const BottomArgs<T> &bottom_args(const ow3d_tank *t);
+template <> inline const BottomArgs<double> &bottom_args<double>(const ow3d_tank *t) { return t->bot64; }
+template <> inline const BottomArgs<float> &bottom_args<float>(const ow3d_tank *t) { return t->bot32; }
+
+template <typename T>
+int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, hipStream_t s) {
+    MgLevel<T> &L = *mg[k];
+    if (k == t->bot_top) {
+        hipLaunchKernelGGL((k_mg_bottom<T>), dim3(1), dim3(512), 0, s, bottom_args<T>(t));
+        return 0;
+    }
+    if (k == 0) {
+        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(1), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
+        return 0;
+    }
+    MgLevel<T> &C = *mg[k - 1];
+    if (smooth(t, L, t->cfg.nu1, s)) return -1;
+    if (mg_launch<T>(L.args, 0, 0, L.x.p, L.b.p, L.r.p, s)) return -1;
+    RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p);
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
+    if (vcycle(t, mg, k - 1, s)) return -1;
+    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks(L.g.n)), dim3(256), 0, s, G, (const T *)C.x.p, L.x.p);
+    if (smooth(t, L, t->cfg.nu2, s)) return -1;
+    return 0;
+}
+
+// The preconditioner body in precision T: input r (FP64) -> finest-level right-hand side, V cycles from
+// x = 0.  Only kernel launches (no memset / memcpy nodes), so the sequence can be captured in a hipGraph.
+template <typename T>
+int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, const double *r, hipStream_t s) {
+    const size_t n = t->g.n;
+    const int top = t->nlev - 1;
+    MgLevel<T> &F = *mg[top];
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p);
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, T(0), F.x.p);
+    for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
+        if (t->nlev == 1 && it > 0) break;
+        if (vcycle<T>(t, mg, top, s)) return -1;
+    }
+    return 0;
+}
+
+}  // namespace mg
+}  // namespace ow3d_host

@@ -1,0 +1,10 @@
+// Multigrid preconditioner in FP32 (level operators, smoother, transfers, bottom solve).
+#include "ow3d_mg.hpp"
+
+namespace ow3d_host {
+
+int mg_build32(ow3d_tank *t) { return mg::mg_build<float>(t, t->mg32); }
+void mg_fill_bottom32(ow3d_tank *t) { mg::fill_bottom<float>(t, t->mg32, t->bot32); }
+int precond_body32(ow3d_tank *t, const double *r, hipStream_t s) { return mg::precond_body<float>(t, t->mg32, r, s); }
+
+}  // namespace ow3d_host
