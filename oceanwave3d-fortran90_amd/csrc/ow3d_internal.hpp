@@ -51,7 +51,7 @@ struct DevBuf {
 template <typename T>
 struct MgLevel {
     Grid g;
-    DevBuf<T> sx, sy, sz, zs, col, x, b, r, minv;
+    DevBuf<T> sx, sy, sz, zs, col, x, x2, b, r, minv;   // x2: second iterate buffer of out-of-place sweeps
     DevBuf<double> xf, yf, zf;   // level coordinates (restriction / prolongation weights)
     LvArgs<T> args{};
 };
@@ -100,6 +100,7 @@ struct ow3d_tank {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool use_graph = true;
     hipGraphExec_t pg_exec = nullptr;   // captured preconditioner (V cycles)
+    const void *mg_out = nullptr;       // finest-level buffer holding the preconditioner result
     ~ow3d_tank() {
         if (pg_exec) (void)hipGraphExecDestroy(pg_exec);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -120,6 +121,6 @@ int mg_build64(ow3d_tank *t);
 int mg_build32(ow3d_tank *t);
 void mg_fill_bottom64(ow3d_tank *t);
 void mg_fill_bottom32(ow3d_tank *t);
-int precond_body64(ow3d_tank *t, const double *r, hipStream_t s);
-int precond_body32(ow3d_tank *t, const double *r, hipStream_t s);
+int precond_body64(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
+int precond_body32(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
 }  // namespace ow3d_host

@@ -16,33 +16,6 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int cclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // ---------------------------------------------------------------------------------------------
-// Sigma-transformation coefficients at one point (DetermineTransformationConstantsArray.f90:143-205)
-// ---------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void sigma_coef3(T s1, T d, T hx, T hy, T hs, T Ex, T Ey, T Es, T &s2, T &s3, T &s4, T &s5) {
-    const T one = T(1), two = T(2);
-    s2 = ((one - s1) * hx - s1 * Ex) / d;
-    s3 = -(two * (one - s1) * (hx * hx + hy * hy) / (d * d))
-         + two * (two * s1 - one) * (Ex * hx + Ey * hy) / (d * d)
-         + (one - s1) * hs / d
-         + two * s1 * (Ex * Ex + Ey * Ey) / (d * d)
-         - s1 * Es / d;
-    s4 = ((one - s1) * hy - s1 * Ey) / d;
-    s5 = one / d;
-}
-template <typename T>
-__device__ __forceinline__ void sigma_coef2(T s1, T d, T hx, T hs, T Ex, T Es, T &s2, T &s3, T &s5) {
-    const T one = T(1), two = T(2);
-    s2 = ((one - s1) * hx - s1 * Ex) / d;
-    s3 = -(two * (one - s1) * (hx * hx) / (d * d))
-         + two * (two * s1 - one) * (Ex * hx) / (d * d)
-         + (one - s1) * hs / d
-         + two * s1 * (Ex * Ex) / (d * d)
-         - s1 * Es / d;
-    s5 = one / d;
-}
-
-// ---------------------------------------------------------------------------------------------
 // High-order matvec  (BuildLinearSystem.f90:1-290)
 // ---------------------------------------------------------------------------------------------
 struct MvArgs {
@@ -635,14 +608,19 @@ struct LvArgs {
     int nxg, nyg, nzg, nxy, gx, gy, dim3;
     const T *sx, *sy, *sz;  // 3-point stencil rows [N][2][3]
     const T *zs;            // sigma of each level (0 on the ghost level)
-    const T *col;           // 4 planes: h, h_x, h_y, h_xx+h_yy
+    const T *col;           // 6 planes: 1/h, h_x, h_y, h_x/h, h_y/h, -2(h_x^2+h_y^2)/h^2 + (h_xx+h_yy)/h
 };
 
+// Coefficients of the linearised (eta = 0) level operator at sigma level k: with eta = 0 the sigma
+// coefficients (DetermineTransformationConstantsArray.f90:190-201) are (1 - sigma) times per-column
+// constants, precomputed at set-up in L.col.
 template <typename T>
 __device__ __forceinline__ void low_coefs(const LvArgs<T> &L, int p, int k, T &s2, T &s3, T &s4, T &s5) {
-    const T d = L.col[p], hx = L.col[L.nxy + p], hy = L.col[2 * L.nxy + p], hs = L.col[3 * L.nxy + p];
-    if (L.dim3) sigma_coef3<T>(L.zs[k], d, hx, hy, hs, T(0), T(0), T(0), s2, s3, s4, s5);
-    else { sigma_coef2<T>(L.zs[k], d, hx, hs, T(0), T(0), s2, s3, s5); s4 = T(0); }
+    const T om = T(1) - L.zs[k];
+    s2 = om * L.col[3 * L.nxy + p];
+    s4 = om * L.col[4 * L.nxy + p];
+    s3 = om * L.col[5 * L.nxy + p];
+    s5 = L.col[p];
 }
 
 // Zebra z-line relaxation of column (i, j): exact tridiagonal solve in sigma of the level operator's
@@ -914,9 +892,10 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // tile and of the one-column ring around it (the ring's red columns get exactly the values their owning
 // blocks compute, since a red column reads only black neighbours, which are not modified in the red
 // half-sweep), then the black columns of the tile against those red values, and writes the tile back.
-// Same arithmetic as a red launch followed by a black launch; x is read and written once.
+// Same arithmetic as a red launch followed by a black launch; x is read once and written once, out of
+// place (xin -> xout): a block's staged neighbours must not change while other blocks write results.
 template <typename T, int NZ>
-__global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
+__global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
     constexpr int TX = 32, TY = 8, H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 128;
     __shared__ T sX[WX * WY * NZP];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
@@ -924,7 +903,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, T *__restrict_
     const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
     const int lox = cmax(i0 - H, 0), loy = d3 ? cmax(j0 - H, 0) : 0;
     const int wcols = cmin(ie + H, nxg) - lox, wrows = d3 ? cmin(je + H, nyg) - loy : 1;
-    stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
     auto own = [&](int ii, int jj) -> T * { return sX + ((jj - loy) * WX + (ii - lox)) * NZP; };
@@ -951,7 +930,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, T *__restrict_
     }
     __syncthreads();
     for (int r = 0; r < je - j0; r++) {
-        T *dst = x + ((size_t)(j0 + r) * nxg + i0) * NZ;
+        T *dst = xout + ((size_t)(j0 + r) * nxg + i0) * NZ;
         for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
             const int c = e / NZ, k = e - c * NZ;
             dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];

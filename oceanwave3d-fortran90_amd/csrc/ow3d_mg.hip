@@ -17,7 +17,7 @@ int mg_setup(ow3d_tank *t) {
     int rc = c.mg_precision == 0 ? mg_build64(t) : mg_build32(t);
     if (rc) return -1;
     // levels small enough for the one-workgroup bottom kernel (OW3D_MG_BOTTOM: point limit, 0 = off)
-    long lim = 50000;
+    long lim = 0;
     if (const char *e = std::getenv("OW3D_MG_BOTTOM")) lim = std::atol(e);
     if (const char *e = std::getenv("OW3D_MG_FUSED_RB")) t->fused_rb = e[0] != '0';
     t->bot_top = -1;
@@ -40,7 +40,7 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
         if (!t->pg_exec) {
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, s) : precond_body32(t, r, s));
+            int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s));
             hipError_t e = hipStreamEndCapture(s, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return -1; }
             HIPCHK(e);
@@ -49,13 +49,12 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
         }
         HIPCHK(hipGraphLaunch(t->pg_exec, s));
     } else {
-        if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, s) : precond_body32(t, r, s))) return -1;
+        if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s))) return -1;
     }
-    const int top = t->nlev - 1;
     if (t->cfg.mg_precision == 0)
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg64[top]->x.p, u);
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg_out, u);
     else
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg32[top]->x.p, u);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg_out, u);
     return 0;
 }
 

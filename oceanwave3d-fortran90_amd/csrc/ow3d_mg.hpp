@@ -16,10 +16,6 @@ void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T 
             dim3 grid((L.nxg + 255) / 256, 1);
             hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
         }
-    } else if (what == 2) {
-        const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-        dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
-        hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ>), grid, dim3(128), 0, s, L, x, b);
     } else {
         const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
         constexpr size_t lds = (size_t)(34 * 10 + 32 * 8) * (NZ | 1) * sizeof(T);
@@ -162,13 +158,16 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         std::vector<double> sy = g.sy;
         if (g.ny == 1) sy.assign(6, 0.0);
         std::vector<double> zs(g.z); zs[0] = 0.0;
-        std::vector<double> col((size_t)4 * g.nxy);
+        std::vector<double> col((size_t)6 * g.nxy);
         for (int p = 0; p < g.nxy; p++) {
-            col[p] = g.h[p]; col[g.nxy + p] = g.hx[p]; col[2 * g.nxy + p] = g.ny > 1 ? g.hy[p] : 0.0;
-            col[3 * g.nxy + p] = g.hxx[p] + (g.ny > 1 ? g.hyy[p] : 0.0);
+            const double d = g.h[p], hx = g.hx[p], hy = g.ny > 1 ? g.hy[p] : 0.0;
+            const double hs = g.hxx[p] + (g.ny > 1 ? g.hyy[p] : 0.0);
+            col[p] = 1.0 / d; col[g.nxy + p] = hx; col[2 * g.nxy + p] = hy;
+            col[3 * g.nxy + p] = hx / d; col[4 * g.nxy + p] = hy / d;
+            col[5 * g.nxy + p] = -(2.0 * (hx * hx + hy * hy) / (d * d)) + hs / d;
         }
         HIPCHK(up(L->sx, g.sx)); HIPCHK(up(L->sy, sy)); HIPCHK(up(L->sz, g.sz)); HIPCHK(up(L->zs, zs)); HIPCHK(up(L->col, col));
-        HIPCHK(L->x.alloc(g.n)); HIPCHK(L->b.alloc(g.n)); HIPCHK(L->r.alloc(g.n));
+        HIPCHK(L->x.alloc(g.n)); HIPCHK(L->x2.alloc(g.n)); HIPCHK(L->b.alloc(g.n)); HIPCHK(L->r.alloc(g.n));
         HIPCHK(L->xf.alloc(g.nxg)); HIPCHK(L->yf.alloc(g.nyg)); HIPCHK(L->zf.alloc(g.nzg));
         HIPCHK(hipMemcpy(L->xf.p, g.x.data(), g.nxg * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(L->yf.p, g.y.data(), g.nyg * 8, hipMemcpyHostToDevice));
@@ -218,25 +217,47 @@ void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, Bot
 }
 
 template <typename T>
-void ghost_pass(MgLevel<T> &L, hipStream_t s) {
+void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
     const Grid &g = L.g;
     const int nb = (g.ny > 1 ? 2 * g.nxg + 2 * (g.nyg - 2) : 2) * g.nzg;
-    hipLaunchKernelGGL((k_mg_ghost_p<T>), dim3((nb + 255) / 256), dim3(256), 0, s, L.args, L.x.p, (const T *)L.b.p);
+    hipLaunchKernelGGL((k_mg_ghost_p<T>), dim3((nb + 255) / 256), dim3(256), 0, s, L.args, x, (const T *)L.b.p);
 }
 
+// Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
+// under it while other blocks write their results).
+template <typename T, int NZ>
+void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
+    const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
+    dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
+    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ>), grid, dim3(128), 0, s, L, xin, xout, b);
+}
 template <typename T>
-int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, hipStream_t s) {
-    ghost_pass(L, s);
+int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
+    switch (L.nzg) {
+#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, s); return 0;
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    return -1;
+}
+
+// ghost pass, nu x (zebra sweep, ghost pass) on the level's current iterate `cur` (L.x or L.x2; a fused
+// sweep moves it to the other buffer).
+template <typename T>
+int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s) {
+    ghost_pass(L, cur, s);
     for (int it = 0; it < nu; it++) {
         if (t->fused_rb) {
-            if (mg_launch<T>(L.args, 2, 0, L.x.p, L.b.p, nullptr, s)) return -1;
+            T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
+            if (rb_launch<T>(L.args, cur, nxt, L.b.p, s)) return -1;
+            cur = nxt;
         } else {
-            if (mg_launch<T>(L.args, 1, 0, L.x.p, L.b.p, nullptr, s)) return -1;
-            if (mg_launch<T>(L.args, 1, 1, L.x.p, L.b.p, nullptr, s)) return -1;
+            if (mg_launch<T>(L.args, 1, 0, cur, L.b.p, nullptr, s)) return -1;
+            if (mg_launch<T>(L.args, 1, 1, cur, L.b.p, nullptr, s)) return -1;
         }
-        ghost_pass(L, s);
+        ghost_pass(L, cur, s);
     }
-    (void)t;
     return 0;
 }
 
@@ -244,41 +265,49 @@ template <typename T> const BottomArgs<T> &bottom_args(const ow3d_tank *t);
 template <> inline const BottomArgs<double> &bottom_args<double>(const ow3d_tank *t) { return t->bot64; }
 template <> inline const BottomArgs<float> &bottom_args<float>(const ow3d_tank *t) { return t->bot32; }
 
+// V cycle on level k from its current iterate `cur` (on entry L.x for every level below the finest);
+// on return `cur` points at the buffer holding the result.
 template <typename T>
-int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, hipStream_t s) {
+int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T *&cur, hipStream_t s) {
     MgLevel<T> &L = *mg[k];
-    if (k == t->bot_top) {
+    if (k == t->bot_top) {   // in place on L.x
         hipLaunchKernelGGL((k_mg_bottom<T>), dim3(1), dim3(512), 0, s, bottom_args<T>(t));
+        cur = L.x.p;
         return 0;
     }
     if (k == 0) {
         hipLaunchKernelGGL((k_mg_coarse<T>), dim3(1), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
+        cur = L.x.p;
         return 0;
     }
     MgLevel<T> &C = *mg[k - 1];
-    if (smooth(t, L, t->cfg.nu1, s)) return -1;
-    if (mg_launch<T>(L.args, 0, 0, L.x.p, L.b.p, L.r.p, s)) return -1;
+    if (smooth(t, L, t->cfg.nu1, cur, s)) return -1;
+    if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p);
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
-    if (vcycle(t, mg, k - 1, s)) return -1;
-    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks(L.g.n)), dim3(256), 0, s, G, (const T *)C.x.p, L.x.p);
-    if (smooth(t, L, t->cfg.nu2, s)) return -1;
+    T *ccur = C.x.p;
+    if (vcycle(t, mg, k - 1, ccur, s)) return -1;
+    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks(L.g.n)), dim3(256), 0, s, G, (const T *)ccur, cur);
+    if (smooth(t, L, t->cfg.nu2, cur, s)) return -1;
     return 0;
 }
 
 // The preconditioner body in precision T: input r (FP64) -> finest-level right-hand side, V cycles from
 // x = 0.  Only kernel launches (no memset / memcpy nodes), so the sequence can be captured in a hipGraph.
+// Returns the buffer holding the result in *out.
 template <typename T>
-int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, const double *r, hipStream_t s) {
+int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, const double *r, const void **out, hipStream_t s) {
     const size_t n = t->g.n;
     const int top = t->nlev - 1;
     MgLevel<T> &F = *mg[top];
     hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p);
     hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, T(0), F.x.p);
+    T *cur = F.x.p;
     for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
         if (t->nlev == 1 && it > 0) break;
-        if (vcycle<T>(t, mg, top, s)) return -1;
+        if (vcycle<T>(t, mg, top, cur, s)) return -1;
     }
+    *out = cur;
     return 0;
 }
 

@@ -5,6 +5,8 @@ namespace ow3d_host {
 
 int mg_build32(ow3d_tank *t) { return mg::mg_build<float>(t, t->mg32); }
 void mg_fill_bottom32(ow3d_tank *t) { mg::fill_bottom<float>(t, t->mg32, t->bot32); }
-int precond_body32(ow3d_tank *t, const double *r, hipStream_t s) { return mg::precond_body<float>(t, t->mg32, r, s); }
+int precond_body32(ow3d_tank *t, const double *r, const void **out, hipStream_t s) {
+    return mg::precond_body<float>(t, t->mg32, r, out, s);
+}
 
 }  // namespace ow3d_host
