@@ -611,44 +611,110 @@ struct LvArgs {
     const T *col;           // 6 planes: 1/h, h_x, h_y, h_x/h, h_y/h, -2(h_x^2+h_y^2)/h^2 + (h_xx+h_yy)/h
 };
 
+// Level coefficients as the column kernels read them: 3-point stencil rows sx(i), sy(j), sz(k) (first then
+// second derivative, 6 values), sigma levels zs(k), and the per-column planes col(plane, i, j).
+// CoefG reads the level arrays in HBM; CoefS reads copies staged in LDS for a window of columns.
+template <typename T>
+struct CoefG {
+    const LvArgs<T> &L;
+    __device__ __forceinline__ const T *sx(int i) const { return L.sx + i * 6; }
+    __device__ __forceinline__ const T *sy(int j) const { return L.sy + j * 6; }
+    __device__ __forceinline__ const T *sz(int k) const { return L.sz + k * 6; }
+    __device__ __forceinline__ T zs(int k) const { return L.zs[k]; }
+    __device__ __forceinline__ T col(int pl, int i, int j) const { return L.col[(size_t)pl * L.nxy + i + (size_t)L.nxg * j]; }
+};
+template <typename T>
+struct CoefS {
+    const T *sx_, *sy_, *sz_, *zs_, *col_;
+    int ib, jb, nx, np;   // window origin, row length and plane size of the staged column planes
+    __device__ __forceinline__ const T *sx(int i) const { return sx_ + (i - ib) * 6; }
+    __device__ __forceinline__ const T *sy(int j) const { return sy_ + (j - jb) * 6; }
+    __device__ __forceinline__ const T *sz(int k) const { return sz_ + k * 6; }
+    __device__ __forceinline__ T zs(int k) const { return zs_[k]; }
+    __device__ __forceinline__ T col(int pl, int i, int j) const { return col_[pl * np + (j - jb) * nx + (i - ib)]; }
+};
+
+// LDS sizes (elements) of the staged coefficients of a window of MW x MH columns and NZ levels.
+template <int MW, int MH, int NZ>
+struct CoefLds {
+    static constexpr int SX = MW * 6, SY = MH * 6, SZ = NZ * 6, ZS = NZ, COL = 6 * MW * MH;
+    static constexpr int TOTAL = SX + SY + SZ + ZS + COL;
+};
+
+// Stages the coefficients of the window [lox, lox+wcols) x [loy, loy+wrows) (row length MW) into buf
+// (CoefLds<MW, MH, NZ>::TOTAL elements), all loads issued before the first store; returns the accessor.
+template <typename T, int NZ, int MW, int MH, int NT>
+__device__ __forceinline__ CoefS<T> stage_coefs(const LvArgs<T> &L, int lox, int loy, int wcols, int wrows, T *buf) {
+    using S = CoefLds<MW, MH, NZ>;
+    constexpr int PER = (S::TOTAL + NT - 1) / NT;
+    T v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int e = (int)threadIdx.x + q * NT;
+        T val = T(0);
+        if (e < S::SX) {
+            if (e / 6 < wcols) val = L.sx[lox * 6 + e];
+        } else if ((e -= S::SX) < S::SY) {
+            if (e / 6 < wrows) val = L.sy[loy * 6 + e];
+        } else if ((e -= S::SY) < S::SZ) {
+            val = L.sz[e];
+        } else if ((e -= S::SZ) < S::ZS) {
+            val = L.zs[e];
+        } else if ((e -= S::ZS) < S::COL) {
+            const int pl = e / (MW * MH), c = e - pl * (MW * MH), r = c / MW, cc = c - r * MW;
+            if (r < wrows && cc < wcols) val = L.col[(size_t)pl * L.nxy + (size_t)(loy + r) * L.nxg + lox + cc];
+        }
+        v[q] = val;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < S::TOTAL) buf[e] = v[q];
+    }
+    CoefS<T> C;
+    C.sx_ = buf; C.sy_ = buf + S::SX; C.sz_ = buf + S::SX + S::SY; C.zs_ = buf + S::SX + S::SY + S::SZ;
+    C.col_ = buf + S::SX + S::SY + S::SZ + S::ZS;
+    C.ib = lox; C.jb = loy; C.nx = MW; C.np = MW * MH;
+    return C;
+}
+
 // Coefficients of the linearised (eta = 0) level operator at sigma level k: with eta = 0 the sigma
 // coefficients (DetermineTransformationConstantsArray.f90:190-201) are (1 - sigma) times per-column
-// constants, precomputed at set-up in L.col.
-template <typename T>
-__device__ __forceinline__ void low_coefs(const LvArgs<T> &L, int p, int k, T &s2, T &s3, T &s4, T &s5) {
-    const T om = T(1) - L.zs[k];
-    s2 = om * L.col[3 * L.nxy + p];
-    s4 = om * L.col[4 * L.nxy + p];
-    s3 = om * L.col[5 * L.nxy + p];
-    s5 = L.col[p];
+// constants, precomputed at set-up (planes 3, 4, 5 and 0 of col).
+template <typename T, class CA>
+__device__ __forceinline__ void low_coefs(const CA &C, int i, int j, int k, T &s2, T &s3, T &s4, T &s5) {
+    const T om = T(1) - C.zs(k);
+    s2 = om * C.col(3, i, j);
+    s4 = om * C.col(4, i, j);
+    s3 = om * C.col(5, i, j);
+    s5 = C.col(0, i, j);
 }
 
 // Zebra z-line relaxation of column (i, j): exact tridiagonal solve in sigma of the level operator's
 // rows with the neighbouring columns frozen (read through X(k, i, j)); the new column goes to out[0..NZ).
 // Couplings to same-colour columns (|di| or |dj| = 2, from the corner-shifted first pass on the sigma
 // ghost level) are left out, so that all columns of one colour can be relaxed at once.
-template <typename T, int NZ, class XA>
-__device__ __forceinline__ void zebra_column(const LvArgs<T> &L, int i, int j, const XA &X, const T *bc, T *out) {
+template <typename T, int NZ, class XA, class CA>
+__device__ __forceinline__ void zebra_column(const LvArgs<T> &L, const CA &C, int i, int j, const XA &X, const T *bc, T *out) {
     const int nxg = L.nxg, nyg = L.nyg;
-    const int p = i + nxg * j;
     const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = L.dim3 ? cclamp(j - 1, 0, nyg - 3) : 0;
-    const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
-    const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
+    const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
+    const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
     int shx = 0; if (wsx == 0) shx = 1; if (wsx + 2 == nxg - 1) shx = -1;
     int shy = 0;
     if (L.dim3) { if (wsy == 0) shy = 1; if (wsy + 2 == nyg - 1) shy = -1; }
-    const T *SXc = L.sx + ((i - shx) * 2) * 3;
-    const T *SYc = L.sy + ((j - shy) * 2) * 3;
+    const T *SXc = C.sx(i - shx);
+    const T *SYc = C.sy(j - shy);
     T lo[NZ], di[NZ], up[NZ], rh[NZ];
     T e2 = T(0);
 #pragma unroll
     for (int k = 0; k < NZ; k++) { lo[k] = T(0); di[k] = T(0); up[k] = T(0); }
     // bottom ghost row
     {
-        const T *Sz = L.sz + (1 * 2) * 3;
+        const T *Sz = C.sz(1);
         T s2, s3, s4, s5;
-        low_coefs(L, p, 1, s2, s3, s4, s5);
-        const T hx = L.col[L.nxy + p], hy = L.dim3 ? L.col[2 * L.nxy + p] : T(0);
+        low_coefs<T>(C, i, j, 1, s2, s3, s4, s5);
+        const T hx = C.col(1, i, j), hy = L.dim3 ? C.col(2, i, j) : T(0);
         const T cz = s5 + hx * s2 + hy * s4;
         T a0 = cz * Sz[0], a1 = cz * Sz[1], a2 = cz * Sz[2], off = T(0);
         for (int m = 0; m < 3; m++) {
@@ -663,9 +729,9 @@ __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, int i, int j, c
     }
 #pragma unroll
     for (int k = 1; k < NZ - 1; k++) {
-        const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
+        const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
         T s2, s3, s4, s5;
-        low_coefs(L, p, k, s2, s3, s4, s5);
+        low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
         const T czz = s2 * s2 + s4 * s4 + s5 * s5;
         T a[3];
         for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
@@ -722,10 +788,9 @@ __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, int i, int j, c
 
 // Residual o = b - L x of column (i, j) of a level (all row kinds: interior, bottom ghost, surface,
 // wall ghost columns, corners), x read through X(k, i, j).
-template <typename T, int NZ, class XA>
-__device__ __forceinline__ void residual_column(const LvArgs<T> &L, int i, int j, const XA &X, const T *bc, T *o) {
+template <typename T, int NZ, class XA, class CA>
+__device__ __forceinline__ void residual_column(const LvArgs<T> &L, const CA &C, int i, int j, const XA &X, const T *bc, T *o) {
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
-    const int p = i + nxg * j;
     const bool wx = L.gx && (i == 0 || i == nxg - 1), wy = L.gy && (j == 0 || j == nyg - 1);
     if (wx && wy) {
 #pragma unroll
@@ -734,19 +799,19 @@ __device__ __forceinline__ void residual_column(const LvArgs<T> &L, int i, int j
         o[0] = bc[0] - X(0, i, j);
         if (wx) {
             const int ib = i == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
-            const T *S = L.sx + (ib * 2) * 3;
+            const T *S = C.sx(ib);
 #pragma unroll
             for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, ws, j) + S[1] * X(k, ws + 1, j)) + S[2] * X(k, ws + 2, j));
         } else {
             const int jb = j == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
-            const T *S = L.sy + (jb * 2) * 3;
+            const T *S = C.sy(jb);
 #pragma unroll
             for (int k = 1; k < NZ; k++) o[k] = bc[k] - ((S[0] * X(k, i, ws) + S[1] * X(k, i, ws + 1)) + S[2] * X(k, i, ws + 2));
         }
     } else {
         const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
-        const T *SX1 = L.sx + (i * 2) * 3, *SX2 = SX1 + 3;
-        const T *SY1 = L.sy + (j * 2) * 3, *SY2 = SY1 + 3;
+        const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
+        const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
         T xc[NZ], dxl[NZ], dyl[NZ];
 #pragma unroll
         for (int k = 0; k < NZ; k++) {
@@ -756,16 +821,16 @@ __device__ __forceinline__ void residual_column(const LvArgs<T> &L, int i, int j
         }
         {
             int sh = 0; if (wsx == 0) sh = 1; if (wsx + 2 == nxg - 1) sh = -1;
-            if (sh) { const T *S = L.sx + ((i - sh) * 2) * 3; dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
+            if (sh) { const T *S = C.sx(i - sh); dxl[0] = (S[0] * X(0, wsx + sh, j) + S[1] * X(0, wsx + sh + 1, j)) + S[2] * X(0, wsx + sh + 2, j); }
             if (d3) {
                 sh = 0; if (wsy == 0) sh = 1; if (wsy + 2 == nyg - 1) sh = -1;
-                if (sh) { const T *S = L.sy + ((j - sh) * 2) * 3; dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
+                if (sh) { const T *S = C.sy(j - sh); dyl[0] = (S[0] * X(0, i, wsy + sh) + S[1] * X(0, i, wsy + sh + 1)) + S[2] * X(0, i, wsy + sh + 2); }
             }
         }
         o[NZ - 1] = bc[NZ - 1] - xc[NZ - 1];
 #pragma unroll
         for (int k = 1; k < NZ - 1; k++) {
-            const T *Z1 = L.sz + (k * 2) * 3, *Z2 = Z1 + 3;
+            const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
             const T dz = (Z1[0] * xc[k - 1] + Z1[1] * xc[k]) + Z1[2] * xc[k + 1];
             const T dzz = (Z2[0] * xc[k - 1] + Z2[1] * xc[k]) + Z2[2] * xc[k + 1];
             const T dxx = (SX2[0] * X(k, wsx, j) + SX2[1] * X(k, wsx + 1, j)) + SX2[2] * X(k, wsx + 2, j);
@@ -773,15 +838,15 @@ __device__ __forceinline__ void residual_column(const LvArgs<T> &L, int i, int j
             const T dxz = (Z1[0] * dxl[k - 1] + Z1[1] * dxl[k]) + Z1[2] * dxl[k + 1];
             const T dyz = (Z1[0] * dyl[k - 1] + Z1[1] * dyl[k]) + Z1[2] * dyl[k + 1];
             T s2, s3, s4, s5;
-            low_coefs(L, p, k, s2, s3, s4, s5);
+            low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
             o[k] = bc[k] - (dxx + dyy + s3 * dz + T(2) * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + s5 * s5) * dzz);
         }
         {
-            const T *Z1 = L.sz + (1 * 2) * 3;
+            const T *Z1 = C.sz(1);
             const T dz = (Z1[0] * xc[0] + Z1[1] * xc[1]) + Z1[2] * xc[2];
             T s2, s3, s4, s5;
-            low_coefs(L, p, 1, s2, s3, s4, s5);
-            const T hx = L.col[L.nxy + p], hy = d3 ? L.col[2 * L.nxy + p] : T(0);
+            low_coefs<T>(C, i, j, 1, s2, s3, s4, s5);
+            const T hx = C.col(1, i, j), hy = d3 ? C.col(2, i, j) : T(0);
             o[0] = bc[0] - ((s5 + hx * s2 + hy * s4) * dz + hx * dxl[1] + hy * dyl[1]);
         }
     }
@@ -850,7 +915,8 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
     const int nxg = L.nxg;
     auto X = [&](int k, int ii, int jj) -> T { return x[((size_t)jj * nxg + ii) * NZ + k]; };
     const size_t p = (size_t)i + (size_t)nxg * j;
-    zebra_column<T, NZ>(L, i, j, X, b + p * NZ, x + p * NZ);
+    const CoefG<T> C{L};
+    zebra_column<T, NZ>(L, C, i, j, X, b + p * NZ, x + p * NZ);
 }
 
 // LDS-tiled zebra z-line relaxation: a block stages a 32 x 8 tile of columns (plus a one-column
@@ -876,7 +942,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
     int i = i0 + 2 * pr;
     if (((i + j) & 1) != colour) i++;
     if (j < je && i < ie)
-        zebra_column<T, NZ>(L, i, j, X, sB + ((j - j0) * TX + (i - i0)) * NZP, sX + ((j - loy) * WX + (i - lox)) * NZP);
+        zebra_column<T, NZ>(L, CoefG<T>{L}, i, j, X, sB + ((j - j0) * TX + (i - i0)) * NZP, sX + ((j - loy) * WX + (i - lox)) * NZP);
     __syncthreads();
     for (int r = 0; r < je - j0; r++) {
         T *dst = x + ((size_t)(j0 + r) * nxg + i0) * NZ;
@@ -887,22 +953,24 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
     }
 }
 
-// One full zebra sweep (red columns, then black columns) in one launch.  A block owns a 32 x 8 tile of
+// One full zebra sweep (red columns, then black columns) in one launch.  A block owns a 32 x TY tile of
 // interior columns and stages the tile with a two-column halo: it first relaxes the red columns of the
 // tile and of the one-column ring around it (the ring's red columns get exactly the values their owning
 // blocks compute, since a red column reads only black neighbours, which are not modified in the red
 // half-sweep), then the black columns of the tile against those red values, and writes the tile back.
 // Same arithmetic as a red launch followed by a black launch; x is read once and written once, out of
 // place (xin -> xout): a block's staged neighbours must not change while other blocks write results.
-template <typename T, int NZ>
-__global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
-    constexpr int TX = 32, TY = 8, H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 128;
+template <typename T, int NZ, int TY>
+__global__ __launch_bounds__(16 * TY) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
+    constexpr int TX = 32, H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 16 * TY;
     __shared__ T sX[WX * WY * NZP];
+    __shared__ T sC[CoefLds<WX, WY, NZ>::TOTAL];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
     const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
     const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
     const int lox = cmax(i0 - H, 0), loy = d3 ? cmax(j0 - H, 0) : 0;
     const int wcols = cmin(ie + H, nxg) - lox, wrows = d3 ? cmin(je + H, nyg) - loy : 1;
+    const CoefS<T> C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
     stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
@@ -916,7 +984,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, const T *__res
             const int ib = i0 - 1;
             const int i = ib + (((ib + j) & 1) != 0 ? 1 : 0) + 2 * t;
             if (i > ie || j < L.gy || j >= (d3 ? nyg - L.gy : 1) || i < L.gx || i >= nxg - L.gx) continue;
-            zebra_column<T, NZ>(L, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
+            zebra_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
         }
     }
     __syncthreads();
@@ -926,7 +994,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_rb(LvArgs<T> L, const T *__res
         const int j = j0 + ty;
         int i = i0 + 2 * pr;
         if (((i + j) & 1) != 1) i++;
-        if (j < je && i < ie) zebra_column<T, NZ>(L, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
+        if (j < je && i < ie) zebra_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
     }
     __syncthreads();
     for (int r = 0; r < je - j0; r++) {
@@ -943,29 +1011,37 @@ __global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict
     ghost_point(L, (int)(blockIdx.x * blockDim.x + threadIdx.x), x, b);
 }
 
-// LDS-tiled residual r = b - L x (16 x 16 columns per block, halo 2 on domain ends as the matvec).
-template <typename T, int NZ>
-__global__ __launch_bounds__(256) void k_mg_residual_t(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
-    constexpr int TX = 16, TY = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1;
-    __shared__ T sX[WX * WY * NZP];
+// LDS-tiled residual r = b - L x (16 x TY columns per block, halo 2 on domain ends as the matvec; x-z
+// problems: one row of 16 * TY columns), level coefficients staged in LDS with the tile.
+template <typename T, int NZ, int TY>
+__global__ __launch_bounds__(16 * TY) void k_mg_residual_t(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
+    constexpr int TX = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1, NT = TX * TY;
+    constexpr int RS2 = TX * TY + 2 * R;       // x-z problems: the tile is one row of TX * TY columns
+    constexpr int LX = WX * WY > RS2 ? WX * WY : RS2;
+    __shared__ T sX[LX * NZP];
+    __shared__ T sC[CoefLds<WX, WY, NZ>::TOTAL > CoefLds<RS2, 1, NZ>::TOTAL ? CoefLds<WX, WY, NZ>::TOTAL : CoefLds<RS2, 1, NZ>::TOTAL];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
     const int i0 = d3 ? (int)blockIdx.x * TX : (int)blockIdx.x * (TX * TY), j0 = d3 ? (int)blockIdx.y * TY : 0;
     const int tw = d3 ? TX : TX * TY;        // columns per tile row
     const int lox = cmax(cmin(i0 - R, nxg - 4), 0);
     const int loy = d3 ? cmax(cmin(j0 - R, nyg - 4), 0) : 0;
     const int wcols = cmin(tw + 2 * R, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
-    const int rowstride = d3 ? WX : (TX * TY + 2 * R);
-    if (d3)
-        stage_window<T, NZ, NZP, WY, WX, WX, 256>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
-    else
-        stage_window<T, NZ, NZP, 1, TX * TY + 2 * R, TX * TY + 2 * R, 256>(sX, x + (size_t)lox * NZ, 0, 1, wcols);
+    const int rowstride = d3 ? WX : RS2;
+    CoefS<T> C;
+    if (d3) {
+        C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
+        stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    } else {
+        C = stage_coefs<T, NZ, RS2, 1, NT>(L, lox, loy, wcols, wrows, sC);
+        stage_window<T, NZ, NZP, 1, RS2, RS2, NT>(sX, x + (size_t)lox * NZ, 0, 1, wcols);
+    }
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * rowstride + (ii - lox)) * NZP + k]; };
     const int tx = d3 ? (int)threadIdx.x % TX : (int)threadIdx.x, ty = d3 ? (int)threadIdx.x / TX : 0;
     const int i = i0 + tx, j = j0 + ty;
-    const bool active = i < nxg && j < nyg && (d3 || ty == 0);
+    const bool active = i < nxg && j < nyg;
     T o[NZ];
-    if (active) residual_column<T, NZ>(L, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, o);
+    if (active) residual_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, o);
     __syncthreads();
     if (active) {
         T *dst = sX + ((j - loy) * rowstride + (i - lox)) * NZP;
@@ -976,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_mg_residual_t(LvArgs<T> L, const T *__r
     const int ie = cmin(i0 + tw, nxg), je = d3 ? cmin(j0 + TY, nyg) : 1;
     for (int rr = 0; rr < je - j0; rr++) {
         T *dst = r + ((size_t)(j0 + rr) * nxg + i0) * NZ;
-        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += blockDim.x) {
+        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
             const int c = e / NZ, k = e - c * NZ;
             dst[e] = sX[((j0 + rr - loy) * rowstride + (i0 + c - lox)) * NZP + k];
         }
@@ -1121,7 +1197,7 @@ __device__ void bottom_sweep(const LvArgs<T> &L, int colour, T *__restrict__ x, 
         int i, j;
         if (!colour_column(L, colour, c, i, j)) continue;
         const size_t p = (size_t)i + (size_t)nxg * j;
-        zebra_column<T, NZ>(L, i, j, X, b + p * NZ, x + p * NZ);
+        zebra_column<T, NZ>(L, CoefG<T>{L}, i, j, X, b + p * NZ, x + p * NZ);
     }
 }
 
@@ -1132,19 +1208,9 @@ __device__ void bottom_residual(const LvArgs<T> &L, const T *__restrict__ x, con
     for (int c = threadIdx.x; c < L.nxy; c += blockDim.x) {
         const int i = c % nxg, j = c / nxg;
         T o[NZ];
-        residual_column<T, NZ>(L, i, j, X, b + (size_t)c * NZ, o);
+        residual_column<T, NZ>(L, CoefG<T>{L}, i, j, X, b + (size_t)c * NZ, o);
 #pragma unroll
         for (int k = 0; k < NZ; k++) r[(size_t)c * NZ + k] = o[k];
-    }
-}
-
-template <typename T>
-__device__ void bottom_dispatch(int what, const LvArgs<T> &L, int colour, T *x, const T *b, T *r) {
-    switch (L.nzg) {
-#define OW3D_BZ(N) case N: if (what == 0) bottom_residual<T, N>(L, x, b, r); else bottom_sweep<T, N>(L, colour, x, b); break;
-        OW3D_BZ(3) OW3D_BZ(4) OW3D_BZ(5) OW3D_BZ(6) OW3D_BZ(7) OW3D_BZ(8) OW3D_BZ(9) OW3D_BZ(10)
-        OW3D_BZ(11) OW3D_BZ(12) OW3D_BZ(13) OW3D_BZ(14)
-#undef OW3D_BZ
     }
 }
 
@@ -1154,26 +1220,27 @@ __device__ void bottom_ghost(const LvArgs<T> &L, T *x, const T *b) {
     for (int t = threadIdx.x; t < nb; t += blockDim.x) ghost_point(L, t, x, b);
 }
 
-template <typename T>
+template <typename T, int NZ>
 __device__ void bottom_smooth(const LvArgs<T> &L, int nu, T *x, const T *b) {
     bottom_ghost(L, x, b);
     __syncthreads();
     for (int it = 0; it < nu; it++) {
-        bottom_dispatch<T>(1, L, 0, x, b, nullptr);
+        bottom_sweep<T, NZ>(L, 0, x, b);
         __syncthreads();
-        bottom_dispatch<T>(1, L, 1, x, b, nullptr);
+        bottom_sweep<T, NZ>(L, 1, x, b);
         __syncthreads();
         bottom_ghost(L, x, b);
         __syncthreads();
     }
 }
 
-template <typename T>
+// All levels 1 .. nlev-1 of the bottom group have NZ sigma levels; level 0 is solved densely.
+template <typename T, int NZ>
 __global__ __launch_bounds__(512) void k_mg_bottom(BottomArgs<T> A) {
     const int top = A.nlev - 1;
     for (int l = top; l >= 1; l--) {
-        bottom_smooth(A.L[l], A.nu1, A.x[l], A.b[l]);
-        bottom_dispatch<T>(0, A.L[l], 0, A.x[l], A.b[l], A.r[l]);
+        bottom_smooth<T, NZ>(A.L[l], A.nu1, A.x[l], A.b[l]);
+        bottom_residual<T, NZ>(A.L[l], A.x[l], A.b[l], A.r[l]);
         __syncthreads();
         const RestrictGeo &G = A.G[l];
         const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
@@ -1191,8 +1258,9 @@ __global__ __launch_bounds__(512) void k_mg_bottom(BottomArgs<T> A) {
         const size_t nf = (size_t)G.nxgf * G.nygf * G.nzgf;
         for (size_t q = threadIdx.x; q < nf; q += blockDim.x) prolong_point(G, q, A.x[l - 1], A.x[l]);
         __syncthreads();
-        bottom_smooth(A.L[l], A.nu2, A.x[l], A.b[l]);
+        bottom_smooth<T, NZ>(A.L[l], A.nu2, A.x[l], A.b[l]);
     }
 }
+
 
 }  // namespace ow3d

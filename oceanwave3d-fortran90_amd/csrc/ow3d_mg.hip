@@ -17,12 +17,17 @@ int mg_setup(ow3d_tank *t) {
     int rc = c.mg_precision == 0 ? mg_build64(t) : mg_build32(t);
     if (rc) return -1;
     // levels small enough for the one-workgroup bottom kernel (OW3D_MG_BOTTOM: point limit, 0 = off)
+    // The group must share one column depth on levels 1 .. bot_top (level 0 is solved densely).
     long lim = 0;
     if (const char *e = std::getenv("OW3D_MG_BOTTOM")) lim = std::atol(e);
     if (const char *e = std::getenv("OW3D_MG_FUSED_RB")) t->fused_rb = e[0] != '0';
     t->bot_top = -1;
-    for (int k = 0; k < t->nlev && k < MG_BOTTOM_MAX; k++)
-        if ((long)t->levels[k].n <= lim) t->bot_top = k;
+    for (int k = 1; k < t->nlev && k < MG_BOTTOM_MAX; k++) {
+        if ((long)t->levels[k].n > lim) break;
+        bool same = true;
+        for (int q = 1; q < k; q++) same &= t->levels[q].nzg == t->levels[k].nzg;
+        if (same) t->bot_top = k;
+    }
     if (c.mg_precision == 0) mg_fill_bottom64(t);
     else mg_fill_bottom32(t);
     return 0;

@@ -6,16 +6,25 @@
 namespace ow3d_host {
 namespace mg {
 
+// LDS bytes of the tiled residual (16 x TY columns) and of the fused sweep (32 x TY columns).
+template <typename T, int NZ, int TY>
+constexpr size_t residual_lds() {
+    constexpr int WX = 20, WY = TY + 4, RS2 = 16 * TY + 4, LX = WX * WY > RS2 ? WX * WY : RS2;
+    constexpr int C3 = CoefLds<WX, WY, NZ>::TOTAL, C2 = CoefLds<RS2, 1, NZ>::TOTAL;
+    return ((size_t)LX * (NZ | 1) + (C3 > C2 ? C3 : C2)) * sizeof(T);
+}
+template <typename T, int NZ, int TY>
+constexpr size_t rb_lds() {
+    return ((size_t)36 * (TY + 4) * (NZ | 1) + CoefLds<36, TY + 4, NZ>::TOTAL) * sizeof(T);
+}
+
 template <typename T, int NZ>
 void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, hipStream_t s) {
     if (what == 0) {
-        if (L.dim3) {
-            dim3 grid((L.nxg + 15) / 16, (L.nyg + 15) / 16);
-            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
-        } else {
-            dim3 grid((L.nxg + 255) / 256, 1);
-            hipLaunchKernelGGL((k_mg_residual_t<T, NZ>), grid, dim3(256), 0, s, L, (const T *)x, b, r);
-        }
+        constexpr int TY = residual_lds<T, NZ, 16>() <= 65536 ? 16 : 8;
+        static_assert(residual_lds<T, NZ, TY>() <= 65536, "residual tile exceeds 64 KiB of LDS");
+        dim3 grid = L.dim3 ? dim3((L.nxg + 15) / 16, (L.nyg + TY - 1) / TY) : dim3((L.nxg + 16 * TY - 1) / (16 * TY), 1);
+        hipLaunchKernelGGL((k_mg_residual_t<T, NZ, TY>), grid, dim3(16 * TY), 0, s, L, (const T *)x, b, r);
     } else {
         const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
         constexpr size_t lds = (size_t)(34 * 10 + 32 * 8) * (NZ | 1) * sizeof(T);
@@ -227,9 +236,11 @@ void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
 // under it while other blocks write their results).
 template <typename T, int NZ>
 void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
+    constexpr int TY = rb_lds<T, NZ, 8>() <= 65536 ? 8 : 4;
+    static_assert(rb_lds<T, NZ, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-    dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + 7) / 8 : 1);
-    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ>), grid, dim3(128), 0, s, L, xin, xout, b);
+    dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + TY - 1) / TY : 1);
+    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TY>), grid, dim3(16 * TY), 0, s, L, xin, xout, b);
 }
 template <typename T>
 int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
@@ -271,7 +282,13 @@ template <typename T>
 int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T *&cur, hipStream_t s) {
     MgLevel<T> &L = *mg[k];
     if (k == t->bot_top) {   // in place on L.x
-        hipLaunchKernelGGL((k_mg_bottom<T>), dim3(1), dim3(512), 0, s, bottom_args<T>(t));
+        switch (L.g.nzg) {
+#define OW3D_NZ(N) case N: hipLaunchKernelGGL((k_mg_bottom<T, N>), dim3(1), dim3(512), 0, s, bottom_args<T>(t)); break;
+            OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+            OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+            default: err() = "bottom level depth outside 3..14"; return -1;
+        }
         cur = L.x.p;
         return 0;
     }
