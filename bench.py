@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
+                    help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
                     help="only run K high-order matvec launches of the workload and a 256 MiB device copy "
                          "(the calibration kernel for HBM counters), then exit")
@@ -90,6 +92,14 @@ def main():
     from ow3d import problems
     if args.profile_matvec:
         profile_matvec(ow3d, problems, args.profile_matvec)
+        return
+    if args.profile_precond:
+        T, cfg, dt = make_tank(ow3d, problems, 1)
+        r = np.random.default_rng(0).standard_normal(T.shape3)
+        for _ in range(args.profile_precond):
+            T.precond_apply(r)
+        print(json.dumps({"precond_applications": args.profile_precond}), flush=True)
+        T.close()
         return
 
     results = {}

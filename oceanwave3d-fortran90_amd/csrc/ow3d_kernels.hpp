@@ -7,6 +7,8 @@
 // phi is read from HBM once per matvec (halo re-reads hit L2 / MALL) and y written once.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <cstdint>
 
 namespace ow3d {
@@ -642,35 +644,44 @@ struct CoefLds {
 };
 
 // Stages the coefficients of the window [lox, lox+wcols) x [loy, loy+wrows) (row length MW) into buf
-// (CoefLds<MW, MH, NZ>::TOTAL elements), all loads issued before the first store; returns the accessor.
+// (CoefLds<MW, MH, NZ>::TOTAL elements) and returns the accessor.  Each segment is one predicated load per
+// element with no branches around it, so all loads of a thread are in flight together.
 template <typename T, int NZ, int MW, int MH, int NT>
 __device__ __forceinline__ CoefS<T> stage_coefs(const LvArgs<T> &L, int lox, int loy, int wcols, int wrows, T *buf) {
     using S = CoefLds<MW, MH, NZ>;
-    constexpr int PER = (S::TOTAL + NT - 1) / NT;
-    T v[PER];
+    constexpr int PX = (S::SX + NT - 1) / NT, PY = (S::SY + NT - 1) / NT, PZ = (S::SZ + S::ZS + NT - 1) / NT;
+    constexpr int PC = (S::COL + NT - 1) / NT;
+    const int t = (int)threadIdx.x;
+    T vx[PX], vy[PY], vz[PZ], vc[PC];
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int e = (int)threadIdx.x + q * NT;
-        T val = T(0);
-        if (e < S::SX) {
-            if (e / 6 < wcols) val = L.sx[lox * 6 + e];
-        } else if ((e -= S::SX) < S::SY) {
-            if (e / 6 < wrows) val = L.sy[loy * 6 + e];
-        } else if ((e -= S::SY) < S::SZ) {
-            val = L.sz[e];
-        } else if ((e -= S::SZ) < S::ZS) {
-            val = L.zs[e];
-        } else if ((e -= S::ZS) < S::COL) {
-            const int pl = e / (MW * MH), c = e - pl * (MW * MH), r = c / MW, cc = c - r * MW;
-            if (r < wrows && cc < wcols) val = L.col[(size_t)pl * L.nxy + (size_t)(loy + r) * L.nxg + lox + cc];
-        }
-        v[q] = val;
+    for (int q = 0; q < PX; q++) {
+        const int e = t + q * NT;
+        vx[q] = (e < S::SX && e / 6 < wcols) ? L.sx[lox * 6 + e] : T(0);
     }
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int e = (int)threadIdx.x + q * NT;
-        if (e < S::TOTAL) buf[e] = v[q];
+    for (int q = 0; q < PY; q++) {
+        const int e = t + q * NT;
+        vy[q] = (e < S::SY && e / 6 < wrows) ? L.sy[loy * 6 + e] : T(0);
     }
+#pragma unroll
+    for (int q = 0; q < PZ; q++) {
+        const int e = t + q * NT;
+        vz[q] = e < S::SZ ? L.sz[e] : (e < S::SZ + S::ZS ? L.zs[e - S::SZ] : T(0));
+    }
+#pragma unroll
+    for (int q = 0; q < PC; q++) {
+        const int e = t + q * NT;
+        const int pl = e / (MW * MH), c = e - pl * (MW * MH), r = c / MW, cc = c - r * MW;
+        vc[q] = (e < S::COL && r < wrows && cc < wcols) ? L.col[(size_t)pl * L.nxy + (size_t)(loy + r) * L.nxg + lox + cc] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PX; q++) { const int e = t + q * NT; if (e < S::SX) buf[e] = vx[q]; }
+#pragma unroll
+    for (int q = 0; q < PY; q++) { const int e = t + q * NT; if (e < S::SY) buf[S::SX + e] = vy[q]; }
+#pragma unroll
+    for (int q = 0; q < PZ; q++) { const int e = t + q * NT; if (e < S::SZ + S::ZS) buf[S::SX + S::SY + e] = vz[q]; }
+#pragma unroll
+    for (int q = 0; q < PC; q++) { const int e = t + q * NT; if (e < S::COL) buf[S::SX + S::SY + S::SZ + S::ZS + e] = vc[q]; }
     CoefS<T> C;
     C.sx_ = buf; C.sy_ = buf + S::SX; C.sz_ = buf + S::SX + S::SY; C.zs_ = buf + S::SX + S::SY + S::SZ;
     C.col_ = buf + S::SX + S::SY + S::SZ + S::ZS;
@@ -694,9 +705,104 @@ __device__ __forceinline__ void low_coefs(const CA &C, int i, int j, int k, T &s
 // rows with the neighbouring columns frozen (read through X(k, i, j)); the new column goes to out[0..NZ).
 // Couplings to same-colour columns (|di| or |dj| = 2, from the corner-shifted first pass on the sigma
 // ghost level) are left out, so that all columns of one colour can be relaxed at once.
+// Tridiagonal solve of a zebra column given its rows (lo, di, up, rh; e2 = third entry of the bottom row).
+template <typename T, int NZ>
+__device__ __forceinline__ void zebra_thomas(T (&lo)[NZ], T (&di)[NZ], T (&up)[NZ], T (&rh)[NZ], T e2, T *out) {
+    {   // eliminate the bottom row's third entry with row 1 (no-op when it is zero)
+        const T f = e2 != T(0) ? e2 / up[1] : T(0);
+        di[0] -= f * lo[1]; up[0] -= f * di[1]; rh[0] -= f * rh[1];
+    }
+    T inv[NZ];   // reciprocal pivots: one division per level
+    inv[0] = T(1) / di[0];
+#pragma unroll
+    for (int k = 1; k < NZ; k++) {
+        const T f = lo[k] * inv[k - 1];
+        di[k] -= f * up[k - 1];
+        rh[k] -= f * rh[k - 1];
+        inv[k] = T(1) / di[k];
+    }
+    T xn = rh[NZ - 1] * inv[NZ - 1];
+    out[NZ - 1] = xn;
+#pragma unroll
+    for (int k = NZ - 2; k >= 0; k--) { xn = (rh[k] - up[k] * xn) * inv[k]; out[k] = xn; }
+}
+
+// zebra_column for a column at least two columns away from every wall (2 <= i <= nxg-3, and the same in
+// y): no clamped or shifted stencil windows, so every neighbour term is fixed at compile time.  Same
+// terms in the same order as the general case.
+template <typename T, int NZ, class XA, class CA>
+__device__ __forceinline__ void zebra_column_interior(const LvArgs<T> &L, const CA &C, int i, int j, const XA &X, const T *bc, T *out) {
+    const bool d3 = L.dim3;
+    const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
+    const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
+    T lo[NZ], di[NZ], up[NZ], rh[NZ];
+    T e2;
+    {
+        const T *Sz = C.sz(1);
+        T s2, s3, s4, s5;
+        low_coefs<T>(C, i, j, 1, s2, s3, s4, s5);
+        const T hx = C.col(1, i, j), hy = d3 ? C.col(2, i, j) : T(0);
+        const T cz = s5 + hx * s2 + hy * s4;
+        T a1 = cz * Sz[1], off = T(0);
+        off += hx * SX1[0] * X(1, i - 1, j);
+        a1 += hx * SX1[1];
+        off += hx * SX1[2] * X(1, i + 1, j);
+        if (d3) {
+            off += hy * SY1[0] * X(1, i, j - 1);
+            a1 += hy * SY1[1];
+            off += hy * SY1[2] * X(1, i, j + 1);
+        }
+        lo[0] = T(0); di[0] = cz * Sz[0]; up[0] = a1; e2 = cz * Sz[2];
+        rh[0] = bc[0] - off;
+    }
+#pragma unroll
+    for (int k = 1; k < NZ - 1; k++) {
+        const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
+        T s2, s3, s4, s5;
+        low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
+        const T czz = s2 * s2 + s4 * s4 + s5 * s5;
+        T a[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
+        T off = T(0);
+        off += SX2[0] * X(k, i - 1, j);
+        a[1] += SX2[1];
+        off += SX2[2] * X(k, i + 1, j);
+#pragma unroll
+        for (int mz = 0; mz < 3; mz++) {
+            const int kk = k - 1 + mz;
+            const T w0 = T(2) * s2 * Z1[mz] * SX1[0], w1 = T(2) * s2 * Z1[mz] * SX1[1], w2 = T(2) * s2 * Z1[mz] * SX1[2];
+            off += w0 * X(kk, i - 1, j);
+            a[mz] += w1;
+            off += w2 * X(kk, i + 1, j);
+        }
+        if (d3) {
+            off += SY2[0] * X(k, i, j - 1);
+            a[1] += SY2[1];
+            off += SY2[2] * X(k, i, j + 1);
+#pragma unroll
+            for (int mz = 0; mz < 3; mz++) {
+                const int kk = k - 1 + mz;
+                const T w0 = T(2) * s4 * Z1[mz] * SY1[0], w1 = T(2) * s4 * Z1[mz] * SY1[1], w2 = T(2) * s4 * Z1[mz] * SY1[2];
+                off += w0 * X(kk, i, j - 1);
+                a[mz] += w1;
+                off += w2 * X(kk, i, j + 1);
+            }
+        }
+        lo[k] = a[0]; di[k] = a[1]; up[k] = a[2];
+        rh[k] = bc[k] - off;
+    }
+    lo[NZ - 1] = T(0); up[NZ - 1] = T(0); di[NZ - 1] = T(1); rh[NZ - 1] = bc[NZ - 1];
+    zebra_thomas<T, NZ>(lo, di, up, rh, e2, out);
+}
+
 template <typename T, int NZ, class XA, class CA>
 __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, const CA &C, int i, int j, const XA &X, const T *bc, T *out) {
     const int nxg = L.nxg, nyg = L.nyg;
+    if (i >= 2 && i <= nxg - 3 && (!L.dim3 || (j >= 2 && j <= nyg - 3))) {
+        zebra_column_interior<T, NZ>(L, C, i, j, X, bc, out);
+        return;
+    }
     const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = L.dim3 ? cclamp(j - 1, 0, nyg - 3) : 0;
     const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
     const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
@@ -717,13 +823,24 @@ __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, const CA &C, in
         const T hx = C.col(1, i, j), hy = L.dim3 ? C.col(2, i, j) : T(0);
         const T cz = s5 + hx * s2 + hy * s4;
         T a0 = cz * Sz[0], a1 = cz * Sz[1], a2 = cz * Sz[2], off = T(0);
+        // branch-free: the own column's weight goes to the matrix, a neighbour's value times its weight
+        // to the right-hand side (the own column is read in place of the neighbour where it is skipped)
+#pragma unroll
         for (int m = 0; m < 3; m++) {
-            if (wsx + m == i) a1 += hx * SX1[m]; else off += hx * SX1[m] * X(1, wsx + m, j);
+            const bool self = wsx + m == i;
+            const T v = X(1, self ? i : wsx + m, j);
+            a1 += self ? hx * SX1[m] : T(0);
+            off += self ? T(0) : hx * SX1[m] * v;
         }
-        if (L.dim3)
+        if (L.dim3) {
+#pragma unroll
             for (int m = 0; m < 3; m++) {
-                if (wsy + m == j) a1 += hy * SY1[m]; else off += hy * SY1[m] * X(1, i, wsy + m);
+                const bool self = wsy + m == j;
+                const T v = X(1, i, self ? j : wsy + m);
+                a1 += self ? hy * SY1[m] : T(0);
+                off += self ? T(0) : hy * SY1[m] * v;
             }
+        }
         di[0] = a0; up[0] = a1; e2 = a2;
         rh[0] = bc[0] - off;
     }
@@ -734,35 +851,52 @@ __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, const CA &C, in
         low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
         const T czz = s2 * s2 + s4 * s4 + s5 * s5;
         T a[3];
+#pragma unroll
         for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
         T off = T(0);
+#pragma unroll
         for (int m = 0; m < 3; m++) {
-            if (wsx + m == i) a[1] += SX2[m]; else off += SX2[m] * X(k, wsx + m, j);
+            const bool self = wsx + m == i;
+            const T v = X(k, self ? i : wsx + m, j);
+            a[1] += self ? SX2[m] : T(0);
+            off += self ? T(0) : SX2[m] * v;
         }
+#pragma unroll
         for (int mz = 0; mz < 3; mz++) {
             const int kk = k - 1 + mz;
             const int sh = (kk == 0) ? shx : 0;
             const T *S1 = (kk == 0) ? SXc : SX1;
+#pragma unroll
             for (int m = 0; m < 3; m++) {
                 const T w = T(2) * s2 * Z1[mz] * S1[m];
                 const int ii = wsx + sh + m;
-                if (ii == i) a[mz] += w;
-                else if (ii - i != 2 && i - ii != 2) off += w * X(kk, ii, j);
+                const bool self = ii == i, skip = self || ii - i == 2 || i - ii == 2;
+                const T v = X(kk, skip ? i : ii, j);
+                a[mz] += self ? w : T(0);
+                off += skip ? T(0) : w * v;
             }
         }
         if (L.dim3) {
+#pragma unroll
             for (int m = 0; m < 3; m++) {
-                if (wsy + m == j) a[1] += SY2[m]; else off += SY2[m] * X(k, i, wsy + m);
+                const bool self = wsy + m == j;
+                const T v = X(k, i, self ? j : wsy + m);
+                a[1] += self ? SY2[m] : T(0);
+                off += self ? T(0) : SY2[m] * v;
             }
+#pragma unroll
             for (int mz = 0; mz < 3; mz++) {
                 const int kk = k - 1 + mz;
                 const int sh = (kk == 0) ? shy : 0;
                 const T *S1 = (kk == 0) ? SYc : SY1;
+#pragma unroll
                 for (int m = 0; m < 3; m++) {
                     const T w = T(2) * s4 * Z1[mz] * S1[m];
                     const int jj = wsy + sh + m;
-                    if (jj == j) a[mz] += w;
-                    else if (jj - j != 2 && j - jj != 2) off += w * X(kk, i, jj);
+                    const bool self = jj == j, skip = self || jj - j == 2 || j - jj == 2;
+                    const T v = X(kk, i, skip ? j : jj);
+                    a[mz] += self ? w : T(0);
+                    off += skip ? T(0) : w * v;
                 }
             }
         }
@@ -770,20 +904,172 @@ __device__ __forceinline__ void zebra_column(const LvArgs<T> &L, const CA &C, in
         rh[k] = bc[k] - off;
     }
     di[NZ - 1] = T(1); rh[NZ - 1] = bc[NZ - 1];
-    if (e2 != T(0)) {
-        const T f = e2 / up[1];
-        di[0] -= f * lo[1]; up[0] -= f * di[1]; rh[0] -= f * rh[1];
+    zebra_thomas<T, NZ>(lo, di, up, rh, e2, out);
+}
+
+// Row k of the zebra line system of column (i, j) (the same rows zebra_column assembles): lo, di, up on
+// levels k-1, k, k+1, the right-hand side with the neighbouring columns moved over, and for the bottom
+// ghost row (k = 0) its third entry e2 (level 2).  nz = column depth.
+template <typename T, class XA, class CA>
+__device__ __forceinline__ void zebra_row(const LvArgs<T> &L, const CA &C, int i, int j, int k, int nz, const XA &X, T bk,
+                                          T &lo, T &di, T &up, T &rh, T &e2) {
+    const int nxg = L.nxg, nyg = L.nyg;
+    const bool d3 = L.dim3;
+    lo = T(0); up = T(0); e2 = T(0);
+    if (k == nz - 1) { di = T(1); rh = bk; return; }
+    const int wsx = cclamp(i - 1, 0, nxg - 3), wsy = d3 ? cclamp(j - 1, 0, nyg - 3) : 0;
+    const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
+    const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
+    if (k == 0) {
+        const T *Sz = C.sz(1);
+        T s2, s3, s4, s5;
+        low_coefs<T>(C, i, j, 1, s2, s3, s4, s5);
+        const T hx = C.col(1, i, j), hy = d3 ? C.col(2, i, j) : T(0);
+        const T cz = s5 + hx * s2 + hy * s4;
+        T a1 = cz * Sz[1], off = T(0);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const bool self = wsx + m == i;
+            const T v = X(1, self ? i : wsx + m, j);
+            a1 += self ? hx * SX1[m] : T(0);
+            off += self ? T(0) : hx * SX1[m] * v;
+        }
+        if (d3) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const bool self = wsy + m == j;
+                const T v = X(1, i, self ? j : wsy + m);
+                a1 += self ? hy * SY1[m] : T(0);
+                off += self ? T(0) : hy * SY1[m] * v;
+            }
+        }
+        di = cz * Sz[0]; up = a1; e2 = cz * Sz[2];
+        rh = bk - off;
+        return;
+    }
+    int shx = 0; if (wsx == 0) shx = 1; if (wsx + 2 == nxg - 1) shx = -1;
+    int shy = 0;
+    if (d3) { if (wsy == 0) shy = 1; if (wsy + 2 == nyg - 1) shy = -1; }
+    const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
+    T s2, s3, s4, s5;
+    low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
+    const T czz = s2 * s2 + s4 * s4 + s5 * s5;
+    T a[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
+    T off = T(0);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const bool self = wsx + m == i;
+        const T v = X(k, self ? i : wsx + m, j);
+        a[1] += self ? SX2[m] : T(0);
+        off += self ? T(0) : SX2[m] * v;
     }
 #pragma unroll
-    for (int k = 1; k < NZ; k++) {
-        const T f = lo[k] / di[k - 1];
-        di[k] -= f * up[k - 1];
-        rh[k] -= f * rh[k - 1];
-    }
-    T xn = rh[NZ - 1] / di[NZ - 1];
-    out[NZ - 1] = xn;
+    for (int mz = 0; mz < 3; mz++) {
+        const int kk = k - 1 + mz;
+        const int sh = (kk == 0) ? shx : 0;
+        const T *S1 = (kk == 0) ? C.sx(i - shx) : SX1;
 #pragma unroll
-    for (int k = NZ - 2; k >= 0; k--) { xn = (rh[k] - up[k] * xn) / di[k]; out[k] = xn; }
+        for (int m = 0; m < 3; m++) {
+            const T w = T(2) * s2 * Z1[mz] * S1[m];
+            const int ii = wsx + sh + m;
+            const bool self = ii == i, skip = self || ii - i == 2 || i - ii == 2;
+            const T v = X(kk, skip ? i : ii, j);
+            a[mz] += self ? w : T(0);
+            off += skip ? T(0) : w * v;
+        }
+    }
+    if (d3) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const bool self = wsy + m == j;
+            const T v = X(k, i, self ? j : wsy + m);
+            a[1] += self ? SY2[m] : T(0);
+            off += self ? T(0) : SY2[m] * v;
+        }
+#pragma unroll
+        for (int mz = 0; mz < 3; mz++) {
+            const int kk = k - 1 + mz;
+            const int sh = (kk == 0) ? shy : 0;
+            const T *S1 = (kk == 0) ? C.sy(j - shy) : SY1;
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const T w = T(2) * s4 * Z1[mz] * S1[m];
+                const int jj = wsy + sh + m;
+                const bool self = jj == j, skip = self || jj - j == 2 || j - jj == 2;
+                const T v = X(kk, i, skip ? j : jj);
+                a[mz] += self ? w : T(0);
+                off += skip ? T(0) : w * v;
+            }
+        }
+    }
+    lo = a[0]; di = a[1]; up = a[2];
+    rh = bk - off;
+}
+
+// zebra_row for a column at least two columns from every wall (no clamped or shifted windows): the
+// column's level-0 value is at c in LDS, x- and y-neighbours at fixed offsets SXS and SYS.  Same terms in
+// the same order as zebra_row.
+template <typename T, int SXS, int SYS, class CA>
+__device__ __forceinline__ void zebra_row_interior(const CA &C, bool d3, int i, int j, int k, int nz, const T *c, T bk,
+                                                   T &lo, T &di, T &up, T &rh, T &e2) {
+    lo = T(0); up = T(0); e2 = T(0);
+    if (k == nz - 1) { di = T(1); rh = bk; return; }
+    const T *SX1 = C.sx(i), *SX2 = SX1 + 3;
+    const T *SY1 = C.sy(j), *SY2 = SY1 + 3;
+    const T *xm = c - SXS, *xp = c + SXS, *ym = c - SYS, *yp = c + SYS;
+    if (k == 0) {
+        const T *Sz = C.sz(1);
+        T s2, s3, s4, s5;
+        low_coefs<T>(C, i, j, 1, s2, s3, s4, s5);
+        const T hx = C.col(1, i, j), hy = d3 ? C.col(2, i, j) : T(0);
+        const T cz = s5 + hx * s2 + hy * s4;
+        T a1 = cz * Sz[1], off = T(0);
+        off += hx * SX1[0] * xm[1];
+        a1 += hx * SX1[1];
+        off += hx * SX1[2] * xp[1];
+        if (d3) {
+            off += hy * SY1[0] * ym[1];
+            a1 += hy * SY1[1];
+            off += hy * SY1[2] * yp[1];
+        }
+        di = cz * Sz[0]; up = a1; e2 = cz * Sz[2];
+        rh = bk - off;
+        return;
+    }
+    const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
+    T s2, s3, s4, s5;
+    low_coefs<T>(C, i, j, k, s2, s3, s4, s5);
+    const T czz = s2 * s2 + s4 * s4 + s5 * s5;
+    T a[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) a[m] = s3 * Z1[m] + czz * Z2[m];
+    T off = T(0);
+    off += SX2[0] * xm[k];
+    a[1] += SX2[1];
+    off += SX2[2] * xp[k];
+#pragma unroll
+    for (int mz = 0; mz < 3; mz++) {
+        const int kk = k - 1 + mz;
+        off += T(2) * s2 * Z1[mz] * SX1[0] * xm[kk];
+        a[mz] += T(2) * s2 * Z1[mz] * SX1[1];
+        off += T(2) * s2 * Z1[mz] * SX1[2] * xp[kk];
+    }
+    if (d3) {
+        off += SY2[0] * ym[k];
+        a[1] += SY2[1];
+        off += SY2[2] * yp[k];
+#pragma unroll
+        for (int mz = 0; mz < 3; mz++) {
+            const int kk = k - 1 + mz;
+            off += T(2) * s4 * Z1[mz] * SY1[0] * ym[kk];
+            a[mz] += T(2) * s4 * Z1[mz] * SY1[1];
+            off += T(2) * s4 * Z1[mz] * SY1[2] * yp[kk];
+        }
+    }
+    lo = a[0]; di = a[1]; up = a[2];
+    rh = bk - off;
 }
 
 // Residual o = b - L x of column (i, j) of a level (all row kinds: interior, bottom ghost, surface,
@@ -953,18 +1239,23 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
     }
 }
 
-// One full zebra sweep (red columns, then black columns) in one launch.  A block owns a 32 x TY tile of
-// interior columns and stages the tile with a two-column halo: it first relaxes the red columns of the
-// tile and of the one-column ring around it (the ring's red columns get exactly the values their owning
-// blocks compute, since a red column reads only black neighbours, which are not modified in the red
-// half-sweep), then the black columns of the tile against those red values, and writes the tile back.
-// Same arithmetic as a red launch followed by a black launch; x is read once and written once, out of
-// place (xin -> xout): a block's staged neighbours must not change while other blocks write results.
-template <typename T, int NZ, int TY>
-__global__ __launch_bounds__(16 * TY) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
-    constexpr int TX = 32, H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 16 * TY;
+// One full zebra sweep (red columns, then black columns) in one launch, out of place (xin -> xout: a
+// block's staged neighbours must not change while other blocks write their results).  A block owns a
+// TX x TY tile of interior columns and stages it with a two-column halo.  Each half-sweep runs in two
+// phases: the rows of all columns of the colour are assembled in parallel (one thread per (column,
+// level) row, into LDS), then one thread per column runs the tridiagonal solve.  The red half-sweep
+// covers the tile and its one-column ring (a red column reads only black neighbours, which do not change
+// in the red half-sweep, so the ring gets exactly the values the owning blocks compute); the black
+// half-sweep covers the tile and reads those red values.  Same arithmetic as a red launch followed by a
+// black launch; x is read once and written once.
+template <typename T, int NZ, int TX, int TY>
+__global__ __launch_bounds__(256) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
+    constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 256;
+    constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
     __shared__ T sX[WX * WY * NZP];
     __shared__ T sC[CoefLds<WX, WY, NZ>::TOTAL];
+    __shared__ T sR[4 * NS * NZ];
+    __shared__ T sE[NS];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
     const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
     const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
@@ -974,29 +1265,50 @@ __global__ __launch_bounds__(16 * TY) void k_mg_zebra_rb(LvArgs<T> L, const T *_
     stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
-    auto own = [&](int ii, int jj) -> T * { return sX + ((jj - loy) * WX + (ii - lox)) * NZP; };
-    // red half-sweep over the tile and its one-column ring (interior columns only)
-    {
-        const int r0 = d3 ? j0 - 1 : 0, nr = d3 ? (je - j0) + 2 : 1;
-        constexpr int PR = (TX + 2) / 2;      // red columns per ring-extended row
-        for (int c = threadIdx.x; c < PR * nr; c += NT) {
-            const int rr = c / PR, t = c - rr * PR, j = r0 + rr;
-            const int ib = i0 - 1;
-            const int i = ib + (((ib + j) & 1) != 0 ? 1 : 0) + 2 * t;
-            if (i > ie || j < L.gy || j >= (d3 ? nyg - L.gy : 1) || i < L.gx || i >= nxg - L.gx) continue;
-            zebra_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
+    const int jlo = d3 ? L.gy : 0, jhi = d3 ? nyg - L.gy : 1;
+    // one half-sweep over the columns of colour COL: red (0) over the ring-extended tile, black (1) over
+    // the tile; slot s of row rr is the t-th column of that colour in the row
+    auto half = [&](auto colour_tag) {
+        constexpr int COL = decltype(colour_tag)::value;
+        constexpr int PER = COL == 0 ? PR : TX / 2;
+        const int r0 = COL == 0 ? (d3 ? j0 - 1 : 0) : j0;
+        const int nr = COL == 0 ? (d3 ? (je - j0) + 2 : 1) : je - j0;
+        const int ib = COL == 0 ? i0 - 1 : i0, iend = COL == 0 ? ie : ie - 1;
+        const int ns = PER * nr;
+        auto slot = [&](int sl, int &i, int &j) -> bool {
+            const int rr = sl / PER, t = sl - rr * PER;
+            j = r0 + rr;
+            i = ib + (((ib + j) & 1) != COL ? 1 : 0) + 2 * t;
+            return i <= iend && j >= jlo && j < jhi && i >= L.gx && i < nxg - L.gx;
+        };
+        for (int task = threadIdx.x; task < ns * NZ; task += NT) {
+            const int sl = task / NZ, k = task - sl * NZ;
+            int i, j;
+            if (!slot(sl, i, j)) continue;
+            T lo, di, up, rh, e2;
+            const T bk = b[((size_t)i + (size_t)nxg * j) * NZ + k];
+            if (i >= 2 && i <= nxg - 3 && (!d3 || (j >= 2 && j <= nyg - 3)))
+                zebra_row_interior<T, NZP, WX * NZP>(C, d3, i, j, k, NZ, sX + ((j - loy) * WX + (i - lox)) * NZP, bk, lo, di, up, rh, e2);
+            else
+                zebra_row<T>(L, C, i, j, k, NZ, X, bk, lo, di, up, rh, e2);
+            T *R = sR + (size_t)sl * 4 * NZ;
+            R[k] = lo; R[NZ + k] = di; R[2 * NZ + k] = up; R[3 * NZ + k] = rh;
+            if (k == 0) sE[sl] = e2;
         }
-    }
-    __syncthreads();
-    // black half-sweep over the tile
-    {
-        const int ty = threadIdx.x / 16, pr = threadIdx.x % 16;
-        const int j = j0 + ty;
-        int i = i0 + 2 * pr;
-        if (((i + j) & 1) != 1) i++;
-        if (j < je && i < ie) zebra_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, own(i, j));
-    }
-    __syncthreads();
+        __syncthreads();
+        for (int sl = threadIdx.x; sl < ns; sl += NT) {
+            int i, j;
+            if (!slot(sl, i, j)) continue;
+            const T *R = sR + (size_t)sl * 4 * NZ;
+            T lo[NZ], di[NZ], up[NZ], rh[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; k++) { lo[k] = R[k]; di[k] = R[NZ + k]; up[k] = R[2 * NZ + k]; rh[k] = R[3 * NZ + k]; }
+            zebra_thomas<T, NZ>(lo, di, up, rh, sE[sl], sX + ((j - loy) * WX + (i - lox)) * NZP);
+        }
+        __syncthreads();
+    };
+    half(std::integral_constant<int, 0>{});
+    half(std::integral_constant<int, 1>{});
     for (int r = 0; r < je - j0; r++) {
         T *dst = xout + ((size_t)(j0 + r) * nxg + i0) * NZ;
         for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
@@ -1159,14 +1471,18 @@ __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__r
     prolong_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, ec, xf);
 }
 
-// Coarsest level: x = Minv b (one block).
+// Coarsest level: x = Minv b with the dense inverse computed at set-up.
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_coarse(int n, const T *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x) {
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        T s = T(0);
-        for (int c = 0; c < n; c++) s += Minv[(size_t)r * n + c] * b[c];
-        x[r] = s;
-    }
+    // one 64-lane wave per row: coalesced reads of the row, fixed-order lane sums and wave reduction
+    const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const T *row = Minv + (size_t)r * n;
+    T s = T(0);
+    for (int c = lane; c < n; c += 64) s += row[c] * b[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[r] = s;
 }
 
 

@@ -13,9 +13,10 @@ constexpr size_t residual_lds() {
     constexpr int C3 = CoefLds<WX, WY, NZ>::TOTAL, C2 = CoefLds<RS2, 1, NZ>::TOTAL;
     return ((size_t)LX * (NZ | 1) + (C3 > C2 ? C3 : C2)) * sizeof(T);
 }
-template <typename T, int NZ, int TY>
+template <typename T, int NZ, int TX, int TY>
 constexpr size_t rb_lds() {
-    return ((size_t)36 * (TY + 4) * (NZ | 1) + CoefLds<36, TY + 4, NZ>::TOTAL) * sizeof(T);
+    constexpr int NS = ((TX + 2) / 2) * (TY + 2);
+    return ((size_t)(TX + 4) * (TY + 4) * (NZ | 1) + CoefLds<TX + 4, TY + 4, NZ>::TOTAL + 4 * NS * NZ + NS) * sizeof(T);
 }
 
 template <typename T, int NZ>
@@ -236,12 +237,14 @@ void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
 // under it while other blocks write their results).
 template <typename T, int NZ>
 void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
-    constexpr int TY = rb_lds<T, NZ, 8>() <= 65536 ? 8 : 4;
-    static_assert(rb_lds<T, NZ, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
+    // the widest tile (32 or 16 columns, 4 rows) whose staged data fits in 48 KiB of LDS
+    constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 49152 ? 32 : 16;
+    static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-    dim3 grid((nxi + 31) / 32, L.dim3 ? (nyi + TY - 1) / TY : 1);
-    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TY>), grid, dim3(16 * TY), 0, s, L, xin, xout, b);
+    dim3 grid((nxi + TX - 1) / TX, L.dim3 ? (nyi + TY - 1) / TY : 1);
+    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY>), grid, dim3(256), 0, s, L, xin, xout, b);
 }
+
 template <typename T>
 int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
     switch (L.nzg) {
@@ -293,7 +296,7 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
         return 0;
     }
     if (k == 0) {
-        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(1), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
+        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
         cur = L.x.p;
         return 0;
     }
