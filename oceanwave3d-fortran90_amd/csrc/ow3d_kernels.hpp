@@ -1248,14 +1248,15 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // in the red half-sweep, so the ring gets exactly the values the owning blocks compute); the black
 // half-sweep covers the tile and reads those red values.  Same arithmetic as a red launch followed by a
 // black launch; x is read once and written once.
-template <typename T, int NZ, int TX, int TY>
-__global__ __launch_bounds__(256) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
-    constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1, NT = 256;
+template <typename T, int NZ, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
+    constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
     __shared__ T sX[WX * WY * NZP];
     __shared__ T sC[CoefLds<WX, WY, NZ>::TOTAL];
     __shared__ T sR[4 * NS * NZ];
     __shared__ T sE[NS];
+    __shared__ T sB[(TX + 2) * (TY + 2) * NZ];   // right-hand sides of the ring-extended tile
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
     const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
     const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
@@ -1263,6 +1264,10 @@ __global__ __launch_bounds__(256) void k_mg_zebra_rb(LvArgs<T> L, const T *__res
     const int wcols = cmin(ie + H, nxg) - lox, wrows = d3 ? cmin(je + H, nyg) - loy : 1;
     const CoefS<T> C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
     stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    // b over the ring-extended tile [i0-1, ie] x [j0-1, je] (clipped to the grid)
+    const int blox = i0 - 1, bloy = d3 ? j0 - 1 : 0;
+    stage_window<T, NZ, NZ, TY + 2, TX + 2, TX + 2, NT>(sB, b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ,
+                                                         d3 ? cmin(je + 1, nyg) - bloy : 1, cmin(ie + 1, nxg) - blox);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
     const int jlo = d3 ? L.gy : 0, jhi = d3 ? nyg - L.gy : 1;
@@ -1286,7 +1291,7 @@ __global__ __launch_bounds__(256) void k_mg_zebra_rb(LvArgs<T> L, const T *__res
             int i, j;
             if (!slot(sl, i, j)) continue;
             T lo, di, up, rh, e2;
-            const T bk = b[((size_t)i + (size_t)nxg * j) * NZ + k];
+            const T bk = sB[((j - bloy) * (TX + 2) + (i - blox)) * NZ + k];
             if (i >= 2 && i <= nxg - 3 && (!d3 || (j >= 2 && j <= nyg - 3)))
                 zebra_row_interior<T, NZP, WX * NZP>(C, d3, i, j, k, NZ, sX + ((j - loy) * WX + (i - lox)) * NZP, bk, lo, di, up, rh, e2);
             else

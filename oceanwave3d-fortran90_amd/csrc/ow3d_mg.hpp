@@ -16,7 +16,8 @@ constexpr size_t residual_lds() {
 template <typename T, int NZ, int TX, int TY>
 constexpr size_t rb_lds() {
     constexpr int NS = ((TX + 2) / 2) * (TY + 2);
-    return ((size_t)(TX + 4) * (TY + 4) * (NZ | 1) + CoefLds<TX + 4, TY + 4, NZ>::TOTAL + 4 * NS * NZ + NS) * sizeof(T);
+    return ((size_t)(TX + 4) * (TY + 4) * (NZ | 1) + CoefLds<TX + 4, TY + 4, NZ>::TOTAL + 4 * NS * NZ + NS +
+            (size_t)(TX + 2) * (TY + 2) * NZ) * sizeof(T);
 }
 
 template <typename T, int NZ>
@@ -237,12 +238,15 @@ void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
 // under it while other blocks write their results).
 template <typename T, int NZ>
 void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
-    // the widest tile (32 or 16 columns, 4 rows) whose staged data fits in 48 KiB of LDS
-    constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 49152 ? 32 : 16;
+    // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
+    constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
     dim3 grid((nxi + TX - 1) / TX, L.dim3 ? (nyi + TY - 1) / TY : 1);
-    hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY>), grid, dim3(256), 0, s, L, xin, xout, b);
+    // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
+    // the line systems per thread); larger levels use 256-thread blocks for occupancy
+    if ((long)grid.x * grid.y < 256) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b);
+    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b);
 }
 
 template <typename T>
