@@ -235,6 +235,193 @@ __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restr
     }
 }
 
+// 3D, one thread per sigma column: block = 16 x TY columns.  The tile and its (alpha+1)-wide halo are
+// staged level-major in LDS (sP[k*PS + column], PS odd), so the 16 lanes of a tile row read consecutive
+// words at every stencil tap (conflict-free) and the column's x- and y-derivatives stay in registers for
+// the mixed derivatives.  Sigma-stencil rows and levels are uniform across the block (scalar loads);
+// the coefficient planes of the tile are staged with the window.  Results are written back through LDS
+// as coalesced row segments.
+template <int NZ, int A, int TY>
+__global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    constexpr int TX = 16, NT = TX * TY, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
+    constexpr int PS = (WX * WY) | 1, NC = TX * TY;
+    __shared__ double sP[NZ * PS];
+    __shared__ double sSX[(TX + 2) * 2 * RA];
+    __shared__ double sSY[(TY + 2) * 2 * RA];
+    __shared__ double sCF[7 * NC];
+    const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
+    const int gxt = (nxg + TX - 1) / TX;
+    const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int i0 = (tile % gxt) * TX, j0 = (tile / gxt) * TY;
+    const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
+    const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
+    const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
+    {
+        // every HBM load of the block first: the phi window (stored level-major), the coefficient planes
+        // of the tile and the stencil rows of its columns and rows
+        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT + NT - 1) / NT, CPT = (7 * NC + NT - 1) / NT;
+        double v[PER], cv[CPT];
+        const double *src = phi + ((size_t)loy * nxg + lox) * NZ;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
+            v[q] = (e < TOT && r < wy && c < wx * NZ) ? src[(size_t)r * nxg * NZ + c] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int pl = e / NC, c = e - pl * NC, ii = i0 + c % TX, jj = j0 + c / TX;
+            cv[q] = (e < 7 * NC && ii < nxg && jj < nyg) ? g.cf[(size_t)pl * nxy + ii + (size_t)nxg * jj] : 0.0;
+        }
+        for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
+            const int r = e / (2 * RA), ii = i0 - 1 + r;
+            if (ii >= 0 && ii < nxg) sSX[e] = g.sx[(size_t)ii * 2 * RA + (e - r * 2 * RA)];
+        }
+        for (int e = threadIdx.x; e < (TY + 2) * 2 * RA; e += NT) {
+            const int r = e / (2 * RA), jj = j0 - 1 + r;
+            if (jj >= 0 && jj < nyg) sSY[e] = g.sy[(size_t)jj * 2 * RA + (e - r * 2 * RA)];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
+            if (e < TOT && r < wy && c < wx * NZ) {
+                const int cc = c / NZ, k = c - cc * NZ;
+                sP[k * PS + r * WX + cc] = v[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            if (e < 7 * NC) sCF[e] = cv[q];
+        }
+    }
+    __syncthreads();
+    auto P = [&](int k, int ii, int jj) -> double { return sP[k * PS + (jj - loy) * WX + (ii - lox)]; };
+    auto SX = [&](int ii) -> const double * { return sSX + (ii - (i0 - 1)) * 2 * RA; };
+    auto SY = [&](int jj) -> const double * { return sSY + (jj - (j0 - 1)) * 2 * RA; };
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, col = threadIdx.x;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = (i < nxg) && (j < nyg);
+    double o[NZ];
+    if (active) {
+        const bool wxl = (i == 0 || i == nxg - 1), wyl = (j == 0 || j == nyg - 1);
+        if (wxl && wyl) {
+#pragma unroll
+            for (int k = 0; k < NZ; k++) o[k] = P(k, i, j);
+        } else if (wxl || wyl) {
+            // wall ghost column: first derivative across the wall at the adjacent node, identity at k = 0
+            const int ib = wxl ? ((i == 0) ? 1 : nxg - 2) : ((j == 0) ? 1 : nyg - 2);
+            const int ws = wxl ? cclamp(ib - A, 0, nxg - RA) : cclamp(ib - A, 0, nyg - RA);
+            const double *S = wxl ? SX(ib) : SY(ib);
+            o[0] = P(0, i, j);
+#pragma unroll
+            for (int k = 1; k < NZ; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) s += S[m] * (wxl ? P(k, ws + m, j) : P(k, i, ws + m));
+                o[k] = s;
+            }
+        } else {
+            double dx[NZ], dy[NZ], pc[NZ];
+            const int wsx = cclamp(i - A, 0, nxg - RA), wsy = cclamp(j - A, 0, nyg - RA);
+            {
+                const double *S1 = SX(i), *S2 = S1 + RA;
+#pragma unroll
+                for (int k = 0; k < NZ; k++) {
+                    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += S1[m] * v; d2 += S2[m] * v; }
+                    dx[k] = d1; o[k] = d2;
+                }
+            }
+            {
+                const double *S1 = SY(j), *S2 = S1 + RA;
+#pragma unroll
+                for (int k = 0; k < NZ; k++) {
+                    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); d1 += S1[m] * v; d2 += S2[m] * v; }
+                    dy[k] = d1; o[k] += d2;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NZ; k++) pc[k] = P(k, i, j);
+            // corner-corrected first pass on the sigma ghost level (ConstructTableCrossDerivatives.f90:173-186)
+            double dxc0 = dx[0], dyc0 = dy[0];
+            {
+                int sh = 0;
+                if (wsx == 0) sh = 1;
+                if (wsx + RA - 1 == nxg - 1) sh = -1;
+                if (sh != 0) {
+                    const double *S = SX(i - sh);
+                    double t = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) t += S[m] * P(0, wsx + sh + m, j);
+                    dxc0 = t;
+                }
+                sh = 0;
+                if (wsy == 0) sh = 1;
+                if (wsy + RA - 1 == nyg - 1) sh = -1;
+                if (sh != 0) {
+                    const double *S = SY(j - sh);
+                    double t = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) t += S[m] * P(0, i, wsy + sh + m);
+                    dyc0 = t;
+                }
+            }
+            const double A2 = sCF[col], B2 = sCF[NC + col], A4 = sCF[2 * NC + col], B4 = sCF[3 * NC + col];
+            const double A3 = sCF[4 * NC + col], B3 = sCF[5 * NC + col], S5 = sCF[6 * NC + col];
+            double obot;
+            {
+                constexpr int wz = (1 - A) < 0 ? 0 : (1 - A);
+                const double *Z1 = g.sz + (1 * 2) * RA;
+                double dz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) dz += Z1[m] * pc[wz + m];
+                // h_x = A2 * d and h_y = A4 * d with d = 1 / S5
+                const double sg = g.zs[1], d = 1.0 / S5, hx = A2 * d, hy = A4 * d;
+                obot = (S5 + hx * (A2 + sg * B2) + hy * (A4 + sg * B4)) * dz + hx * dx[1] + hy * dy[1];
+            }
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const int wz = cclamp(k - A, 0, NZ - RA);
+                const double *Z1 = g.sz + (k * 2) * RA, *Z2 = Z1 + RA;
+                double dz = 0.0, dzz = 0.0, dxz = 0.0, dyz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) {
+                    const double z1 = Z1[m], cv = pc[wz + m];
+                    dz += z1 * cv;
+                    dzz += Z2[m] * cv;
+                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
+                    dyz += z1 * ((wz + m == 0) ? dyc0 : dy[wz + m]);
+                }
+                const double sg = g.zs[k];
+                const double s2 = A2 + sg * B2, s4 = A4 + sg * B4, s3 = A3 + sg * B3;
+                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz + s4 * dyz) + (s2 * s2 + s4 * s4 + S5 * S5) * dzz;
+            }
+            o[0] = obot;
+            o[NZ - 1] = pc[NZ - 1];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NZ; k++) sP[k * PS + (j - loy) * WX + (i - lox)] = o[k];
+    }
+    __syncthreads();
+    {
+        const int ie = cmin(i0 + TX, nxg), je = cmin(j0 + TY, nyg);
+        for (int jj = j0; jj < je; jj++) {
+            double *dst = out + ((size_t)jj * nxg + i0) * NZ;
+            for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
+                const int ii = e / NZ, k = e - ii * NZ;
+                dst[e] = sP[k * PS + (jj - loy) * WX + (i0 + ii - lox)];
+            }
+        }
+    }
+}
+
 // x-z problems (Ny == 1), one thread per grid point: block = 64 columns x NZ levels.
 template <int NZ, int A>
 __global__ __launch_bounds__(256) void k_matvec2p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
@@ -1246,8 +1433,9 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // level) row, into LDS), then one thread per column runs the tridiagonal solve.  The red half-sweep
 // covers the tile and its one-column ring (a red column reads only black neighbours, which do not change
 // in the red half-sweep, so the ring gets exactly the values the owning blocks compute); the black
-// half-sweep covers the tile and reads those red values.  Same arithmetic as a red launch followed by a
-// black launch; x is read once and written once.
+// half-sweep covers the tile and reads those red values.  The wall ghost pass before the sweep is done
+// on the staged window and the one after it for the ghost points next to the tile, so one launch does
+// (ghost pass, red, black, ghost pass) with x read once and written once.
 template <typename T, int NZ, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
@@ -1258,8 +1446,14 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     __shared__ T sE[NS];
     __shared__ T sB[(TX + 2) * (TY + 2) * NZ];   // right-hand sides of the ring-extended tile
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
-    const int i0 = L.gx + (int)blockIdx.x * TX, j0 = d3 ? L.gy + (int)blockIdx.y * TY : 0;
-    const int ie = cmin(i0 + TX, nxg - L.gx), je = d3 ? cmin(j0 + TY, nyg - L.gy) : 1;
+    // tiles of TX x TY interior columns; when a direction's count leaves a one-column remainder the first
+    // tile is one column short instead, so every tile next to a wall has the two columns its ghost
+    // values are computed from
+    const int nxi = nxg - 2 * L.gx, nyi = d3 ? nyg - 2 * L.gy : 1;
+    const int offx = (nxi % TX == 1) ? 1 : 0, offy = (d3 && nyi % TY == 1) ? 1 : 0;
+    const int i0 = L.gx + cmax(0, (int)blockIdx.x * TX - offx), ie = L.gx + cmin(nxi, ((int)blockIdx.x + 1) * TX - offx);
+    const int j0 = d3 ? L.gy + cmax(0, (int)blockIdx.y * TY - offy) : 0;
+    const int je = d3 ? L.gy + cmin(nyi, ((int)blockIdx.y + 1) * TY - offy) : 1;
     const int lox = cmax(i0 - H, 0), loy = d3 ? cmax(j0 - H, 0) : 0;
     const int wcols = cmin(ie + H, nxg) - lox, wrows = d3 ? cmin(je + H, nyg) - loy : 1;
     const CoefS<T> C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
@@ -1271,6 +1465,47 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
     const int jlo = d3 ? L.gy : 0, jhi = d3 ? nyg - L.gy : 1;
+    // value of wall ghost point (ii, jj, k) from the staged interior values (GaussSeidelSpecialPoints.f90 rows:
+    // identity on the sigma ghost level and at corners, the Neumann row relaxed pointwise elsewhere)
+    auto ghost_value = [&](int ii, int jj, int k) -> T {
+        const bool wxg = L.gx && (ii == 0 || ii == nxg - 1), wyg = L.gy && (jj == 0 || jj == nyg - 1);
+        const T bq = b[((size_t)jj * nxg + ii) * NZ + k];
+        if ((wxg && wyg) || k == 0) return bq;
+        T sum = T(0), dg = T(1);
+        if (wxg) {
+            const int ib = ii == 0 ? 1 : nxg - 2, ws = cclamp(ib - 1, 0, nxg - 3);
+            const T *S = C.sx(ib);
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                if (ws + m == ii) dg = S[m];
+                else sum += S[m] * X(k, ws + m, jj);
+            }
+        } else {
+            const int jb = jj == 0 ? 1 : nyg - 2, ws = cclamp(jb - 1, 0, nyg - 3);
+            const T *S = C.sy(jb);
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                if (ws + m == jj) dg = S[m];
+                else sum += S[m] * X(k, ii, ws + m);
+            }
+        }
+        return (bq - sum) / dg;
+    };
+    auto is_ghost = [&](int ii, int jj) -> bool {
+        return (L.gx && (ii == 0 || ii == nxg - 1)) || (L.gy && (jj == 0 || jj == nyg - 1));
+    };
+    // the ghost pass before the sweep, on the window (ghost values depend on interior values only)
+    if (lox == 0 || lox + wcols == nxg || (d3 && (loy == 0 || loy + wrows == nyg))) {
+        for (int e = threadIdx.x; e < wrows * wcols * NZ; e += NT) {
+            const int r = e / (wcols * NZ), rem = e - r * wcols * NZ, c = rem / NZ, k = rem - c * NZ;
+            const int ii = lox + c, jj = loy + r;
+            if (is_ghost(ii, jj)) {
+                const T v = ghost_value(ii, jj, k);
+                sX[((jj - loy) * WX + (ii - lox)) * NZP + k] = v;
+            }
+        }
+        __syncthreads();
+    }
     // one half-sweep over the columns of colour COL: red (0) over the ring-extended tile, black (1) over
     // the tile; slot s of row rr is the t-th column of that colour in the row
     auto half = [&](auto colour_tag) {
@@ -1319,6 +1554,20 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
         for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
             const int c = e / NZ, k = e - c * NZ;
             dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
+        }
+    }
+    // the ghost pass after the sweep for the ghost points this tile owns: wall columns beside its rows,
+    // wall rows beside its columns, and the corners of the tiles at the corners of the domain
+    {
+        const int gi0 = (i0 == L.gx) ? 0 : i0, gi1 = (ie == nxg - L.gx) ? nxg : ie;
+        const int gj0 = d3 ? ((j0 == L.gy) ? 0 : j0) : 0, gj1 = d3 ? ((je == nyg - L.gy) ? nyg : je) : 1;
+        if (gi0 != i0 || gi1 != ie || gj0 != j0 || gj1 != je) {
+            const int w = gi1 - gi0;
+            for (int e = threadIdx.x; e < (gj1 - gj0) * w * NZ; e += NT) {
+                const int r = e / (w * NZ), rem = e - r * w * NZ, c = rem / NZ, k = rem - c * NZ;
+                const int ii = gi0 + c, jj = gj0 + r;
+                if (is_ghost(ii, jj)) xout[((size_t)jj * nxg + ii) * NZ + k] = ghost_value(ii, jj, k);
+            }
         }
     }
 }

@@ -15,8 +15,23 @@ constexpr size_t mv3_lds() {
     return ((size_t)(16 + 2 * R) * (TY + 2 * R) * NZ + 2 * 16 * TY * NZ + 7 * 16 * TY + (16 + 2) * 2 * RA + (TY + 2) * 2 * RA +
             NZ * 2 * RA + NZ) * 8;
 }
+template <int NZ, int A, int TY>
+constexpr size_t mv3c_lds() {
+    constexpr int R = A + 1, RA = 2 * A + 1;
+    return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA + 7 * 16 * TY) * 8;
+}
+template <int NZ, int A>
+int launch_mv3c(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+    constexpr int TY = mv3c_lds<NZ, A, 16>() <= 65536 ? 16 : 8;
+    static_assert(mv3c_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
+    const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
+    hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, t->mv, phi, y);
+    return 0;
+}
 template <int NZ, int A>
 int launch_mv3(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+    static const bool col = [] { const char *e = std::getenv("OW3D_MV"); return !(e && e[0] == 'p'); }();
+    if (col) return launch_mv3c<NZ, A>(t, phi, y, s);
     constexpr int TY = mv3_lds<NZ, A, 8>() <= 65536 ? 8 : 4;
     static_assert(mv3_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
     const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);

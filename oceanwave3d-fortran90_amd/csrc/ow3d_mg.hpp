@@ -242,7 +242,8 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStre
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-    dim3 grid((nxi + TX - 1) / TX, L.dim3 ? (nyi + TY - 1) / TY : 1);
+    const int offx = (nxi % TX == 1) ? 1 : 0, offy = (L.dim3 && nyi % TY == 1) ? 1 : 0;
+    dim3 grid((nxi + offx + TX - 1) / TX, L.dim3 ? (nyi + offy + TY - 1) / TY : 1);
     // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
     if ((long)grid.x * grid.y < 256) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b);
@@ -264,16 +265,19 @@ int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t
 // sweep moves it to the other buffer).
 template <typename T>
 int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s) {
-    ghost_pass(L, cur, s);
-    for (int it = 0; it < nu; it++) {
-        if (t->fused_rb) {
+    if (t->fused_rb) {   // each launch is (ghost pass, red, black, ghost pass)
+        if (nu == 0) ghost_pass(L, cur, s);
+        for (int it = 0; it < nu; it++) {
             T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
             if (rb_launch<T>(L.args, cur, nxt, L.b.p, s)) return -1;
             cur = nxt;
-        } else {
-            if (mg_launch<T>(L.args, 1, 0, cur, L.b.p, nullptr, s)) return -1;
-            if (mg_launch<T>(L.args, 1, 1, cur, L.b.p, nullptr, s)) return -1;
         }
+        return 0;
+    }
+    ghost_pass(L, cur, s);
+    for (int it = 0; it < nu; it++) {
+        if (mg_launch<T>(L.args, 1, 0, cur, L.b.p, nullptr, s)) return -1;
+        if (mg_launch<T>(L.args, 1, 1, cur, L.b.p, nullptr, s)) return -1;
         ghost_pass(L, cur, s);
     }
     return 0;
