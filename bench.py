@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-counter runs of the matvec")
     ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
                     help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
@@ -88,6 +89,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    # HBM-counter runs first, as child processes, before this process touches the GPU
+    tr = None if (args.no_traffic or args.profile_matvec or args.profile_precond) else measure_traffic()
     import ow3d
     from ow3d import problems
     if args.profile_matvec:
@@ -145,10 +148,13 @@ def main():
         "mixed": {"p50_ms": 1e3 * float(np.percentile(mix["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(mix["lat"], 99)),
                   "iters": mix["iters"], "rel_err_eta_vs_fp64": err_eta, "rel_err_phi_vs_fp64": err_phi},
         "speedup_mixed_vs_fp64": t_f64 / t_mix,
-        "roofline": {"bound": "hbm", "kernel": "k_matvec3 (high-order sigma-Laplace matvec)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_matvec3c (high-order sigma-Laplace matvec)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": mv_ms * 1e3, "algorithmic_bytes": b_mv},
     }
+    if tr is not None:
+        out["roofline"]["traffic"] = tr["bytes"]
+        out["roofline"]["traffic_detail"] = tr
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
@@ -168,6 +174,50 @@ def profile_matvec(ow3d, problems, k):
     print(json.dumps({"matvec_launches": k, "avg_ms": ms, "n": n, "ncol": T.nxg * T.nyg,
                       "algorithmic_bytes": 16 * n + 40 * T.nxg * T.nyg, "copy_bytes": 2 * src.numel() * 8}), flush=True)
     T.close()
+
+
+def measure_traffic(timeout=120):
+    """HBM bytes per matvec launch from two rocprofv3 counter runs of `bench.py --profile-matvec 5`
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for the
+    256 MiB device copy the profiling run also performs.  Child processes (the profiler starts the
+    program itself); None when rocprofv3 is unavailable or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    res = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            od = os.path.join(d, counter)
+            cmd = [prof, "--pmc", counter, "-d", od, "-o", "run", "--output-format", "csv", "--",
+                   sys.executable, os.path.abspath(__file__), "--profile-matvec", "5"]
+            try:
+                subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return None
+            path = os.path.join(od, "run_counter_collection.csv")
+            if not os.path.exists(path):
+                return None
+            per = {}
+            for r in csv.DictReader(open(path)):
+                k = (int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size"]))
+                per[k] = per.get(k, 0.0) + float(r["Counter_Value"])
+            mv = sorted(v for k, v in per.items() if "k_matvec3" in k[1])
+            copies = [v for k, v in per.items() if "copyBuffer" in k[1] or "copy" in k[1].lower()]
+            if not mv or not copies:
+                return None
+            copy_kib = max(copies)                         # the 256 MiB device copy
+            expect_kib = 256.0 * 1024                      # it reads 256 MiB and writes 256 MiB
+            res[counter] = {"kib_per_launch": mv[len(mv) // 2], "copy_kib": copy_kib, "scale": expect_kib / copy_kib}
+    fetch = res["FETCH_SIZE"]["kib_per_launch"] * res["FETCH_SIZE"]["scale"] * 1024
+    write = res["WRITE_SIZE"]["kib_per_launch"] * res["WRITE_SIZE"]["scale"] * 1024
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_scale": res["FETCH_SIZE"]["scale"], "write_scale": res["WRITE_SIZE"]["scale"]}
 
 
 def make_tank_state(problems):
