@@ -113,7 +113,9 @@ def main():
         lat = run_steps(T, args.steps, dt)
         E, Pp, W, phi = T.get_state()
         tot, solves, itmin, itmax = T.stats()
-        results[label] = dict(lat=lat, E=E, P=Pp, iters=(tot, solves, itmin, itmax))
+        replays, captures, capture_ms = T.step_stats()
+        results[label] = dict(lat=lat, E=E, P=Pp, iters=(tot, solves, itmin, itmax),
+                              graphs=dict(replays=replays, captures=captures, capture_ms=round(capture_ms, 2)))
         if label == "mixed":
             T.time_matvec(5)
             mv_ms = T.time_matvec(200)
@@ -144,9 +146,10 @@ def main():
                    "dof": dof, "solver": "GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, 8 levels"},
         "fp64": {"value": dof * args.steps / t_f64, "ms_per_step": 1e3 * t_f64 / args.steps,
                  "p50_ms": 1e3 * float(np.percentile(f64["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(f64["lat"], 99)),
-                 "iters": f64["iters"]},
+                 "iters": f64["iters"], "step_graphs": f64["graphs"]},
         "mixed": {"p50_ms": 1e3 * float(np.percentile(mix["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(mix["lat"], 99)),
-                  "iters": mix["iters"], "rel_err_eta_vs_fp64": err_eta, "rel_err_phi_vs_fp64": err_phi},
+                  "iters": mix["iters"], "step_graphs": mix["graphs"], "rel_err_eta_vs_fp64": err_eta,
+                  "rel_err_phi_vs_fp64": err_phi},
         "speedup_mixed_vs_fp64": t_f64 / t_mix,
         "roofline": {"bound": "hbm", "kernel": "k_matvec3c (high-order sigma-Laplace matvec)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -102,6 +102,27 @@ void *ow3d_stream(ow3d_tank *t);
  * HIP events on the handle's stream (ow3d_time_matvec runs them). */
 int ow3d_time_matvec(ow3d_tank *t, int n_launches, double *avg_ms);
 
+/* Many tanks at once (no counterpart in the reference, which advances one tank per process): every
+ * handle's solves run on the device with no host round trip, and from a handle's second step on each
+ * RK4 step is one replay of a captured hipGraph on the handle's stream, so tanks advance concurrently.
+ * ow3d_batch_rk4_steps advances tanks[i] by nsteps steps of dt[i], each tank on its own stream as soon
+ * as its previous step has settled, and returns when all are done (1 if any solve missed its tolerance). */
+int ow3d_batch_rk4_steps(ow3d_tank *const *tanks, int ntanks, int nsteps, const double *dt);
+/* Asynchronous form for request loops: ow3d_rk4_steps_async enqueues the first of nsteps steps and
+ * returns; each ow3d_query call settles completed steps and enqueues the next, returning 1 when all
+ * nsteps are done, 0 while steps remain; ow3d_wait blocks and returns as ow3d_rk4_steps. */
+int ow3d_rk4_steps_async(ow3d_tank *t, int nsteps, double dt);
+int ow3d_query(ow3d_tank *t);
+int ow3d_wait(ow3d_tank *t);
+/* Each stage's solve is enqueued with the iteration slots recent steps used plus a margin; a step in
+ * which a solve used all its slots without stopping is rolled back and replayed with the whole budget,
+ * so results equal those of whole-budget steps.  Stages are captured per (stage, slots).
+ * Replays, graph captures and the host time spent capturing, so far: */
+int ow3d_step_stats(const ow3d_tank *t, long long *replays, long long *captures, double *capture_ms);
+/* Device memory held by a handle, and what ow3d_create would allocate for cfg (admission control). */
+long long ow3d_device_bytes(const ow3d_tank *t);
+long long ow3d_estimate_bytes(const ow3d_config *cfg);
+
 #ifdef __cplusplus
 }
 #endif

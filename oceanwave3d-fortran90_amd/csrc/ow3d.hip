@@ -4,6 +4,9 @@
 // the multigrid preconditioner in ow3d_mg.hip.
 #include "ow3d_internal.hpp"
 
+#include <chrono>
+#include <thread>
+
 std::string &ow3d_host::err() {
     static thread_local std::string e;
     return e;
@@ -11,41 +14,48 @@ std::string &ow3d_host::err() {
 
 extern "C" const char *ow3d_last_error(void) { return ow3d_host::err().c_str(); }
 
+std::atomic<long long> &ow3d_host::device_bytes() {
+    static std::atomic<long long> b{0};
+    return b;
+}
+
 namespace {
 
 // ---------------------------------------------------------------------------------------------
-// Krylov helpers
+// Krylov solves, enqueued on the tank's stream with no host round trip (scalar work in the one-thread
+// kernels of ow3d_kernels.hpp); results are read back by the entry points that need them.
 // ---------------------------------------------------------------------------------------------
 int red_blocks(size_t n) { return (int)std::min<size_t>(RED_BLOCKS, std::max<size_t>(1, (n + 1023) / 1024)); }
 
-// out[0..k-1] (device hv) = V_i . w ; copied to host vals.
 // Krylov kernel dispatch on the number of basis vectors (1 .. MAXK-1).
 #define OW3D_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
     X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
 static_assert(MAXK == 32, "Krylov dispatch covers 1..31 vectors");
 
-int launch_mdot(int k, bool norm, int nb, size_t n, const double *V, const double *w, double *part, hipStream_t s) {
+int launch_mdot(int k, bool norm, int nb, size_t n, const double *V, const double *w, double *part, const int *skip, hipStream_t s) {
     switch (k) {
-#define OW3D_C(K) case K: if (norm) hipLaunchKernelGGL((k_mdot<K, 1, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part); \
-                          else hipLaunchKernelGGL((k_mdot<K, 0, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part); return 0;
+#define OW3D_C(K) case K: if (norm) hipLaunchKernelGGL((k_mdot<K, 1, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part, skip); \
+                          else hipLaunchKernelGGL((k_mdot<K, 0, MAXK>), dim3(nb), dim3(256), 0, s, n, V, w, part, skip); return 0;
         OW3D_K_CASES(OW3D_C)
 #undef OW3D_C
     }
     err() = "Krylov dimension out of range";
     return -1;
 }
-int launch_gs_update_dot(int k, int nb, size_t n, const double *V, const double *h, double *v, double *part, hipStream_t s) {
+int launch_gs_update_dot(int k, int nb, size_t n, const double *V, const double *h, double *v, double *part, const int *skip,
+                         hipStream_t s) {
     switch (k) {
-#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update_dot<K, MAXK>), dim3(nb), dim3(256), 0, s, n, V, h, v, part); return 0;
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update_dot<K, MAXK>), dim3(nb), dim3(256), 0, s, n, V, h, v, part, skip); return 0;
         OW3D_K_CASES(OW3D_C)
 #undef OW3D_C
     }
     err() = "Krylov dimension out of range";
     return -1;
 }
-int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, double sign, double scale, double *v, hipStream_t s) {
+int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, const double *scale, double *v, const int *skip,
+                     hipStream_t s) {
     switch (k) {
-#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update<K>), dim3(nb), dim3(256), 0, s, n, V, h, sign, scale, v); return 0;
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_gs_update<K>), dim3(nb), dim3(256), 0, s, n, V, h, -1.0, 1.0, scale, v, skip); return 0;
         OW3D_K_CASES(OW3D_C)
 #undef OW3D_C
     }
@@ -53,144 +63,81 @@ int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, 
     return -1;
 }
 
-int mdot(ow3d_tank *t, int k, const double *V, const double *w, double *vals, hipStream_t s) {
+// hv[0..k-1] = sum over the partial sums of the last reduction
+void reduce(ow3d_tank *t, int k, int nb, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)t->part.p, t->hv.p, skip);
+}
+
+// Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres): classical
+// Gram-Schmidt with the DGKS second pass when the projection removed more than half of the squared norm.
+// At most m operator applications (the initial residual and m - 1 Arnoldi steps), as the reference's
+// ipar(5) = ipar(6) = m setting.
+int full_slots(const ow3d_tank *t) {
+    if (t->cfg.solver == 0) return std::max(1, t->cfg.gmres_m);
+    const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
+    return std::min(m - 1, MAXK - 2);
+}
+
+// slots: Arnoldi steps enqueued (at most full_slots); stage: RK4 stage (0..3) the slot use is kept for.
+int gmres(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, int stage) {
     const size_t n = t->g.n;
-    const int nb = red_blocks(n);
-    if (launch_mdot(k, false, nb, n, V, w, t->part.p, s)) return -1;
-    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)t->part.p, t->hv.p);
-    HIPCHK(hipMemcpyAsync(vals, t->hv.p, k * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
+    double *V = t->V.p, *w = t->w.p;
+    KrylovState *S = t->ks.p;
+    const int *done = t->done(), *skip2 = t->skip2();
+    const int nbl = grid_blocks(n), nb = red_blocks(n);
+    hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, s, S);
+    if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+    hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
+    if (precond(t, w, V, s)) return -1;
+    if (launch_mdot(1, false, nb, n, V, V, t->part.p, done, s)) return -1;
+    reduce(t, 1, nb, done, s);
+    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, t->cfg.reltol, t->cfg.abstol);
+    hipLaunchKernelGGL(k_scale_state, dim3(nbl), dim3(256), 0, s, n, (const KrylovState *)S, V);
+    for (int k = 1; k <= slots; k++) {
+        double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
+        if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+        if (precond(t, w, vn, s)) return -1;
+        if (launch_mdot(k, true, nb, n, V, vn, t->part.p, done, s)) return -1;
+        reduce(t, k + 1, nb, done, s);
+        hipLaunchKernelGGL(k_gmres_pass1, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, k, m);
+        if (launch_gs_update_dot(k, nb, n, V, t->hv.p, vn, t->part.p, skip2, s)) return -1;
+        reduce(t, k + 1, nb, skip2, s);
+        hipLaunchKernelGGL(k_gmres_pass2, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, k, m);
+        // vn = (vn - V h) / hn with the projections of the last pass
+        if (launch_gs_update(k, nb, n, V, t->hv.p, &S->scale, vn, done, s)) return -1;
+    }
+    hipLaunchKernelGGL(k_gmres_finish, dim3(1), dim3(64), 0, s, S, slots, full_slots(t), stage);
+    hipLaunchKernelGGL(k_basis_combine, dim3(nbl), dim3(256), 0, s, n, (const double *)V, (const KrylovState *)S, x);
     return 0;
 }
 
-double norm2_host(ow3d_tank *t, const double *v, hipStream_t s, int &fail) {
-    double val = 0.0;
-    fail = mdot(t, 1, v, v, &val, s);
-    return std::sqrt(val);
-}
-
-void record_iters(ow3d_tank *t, int it) {
-    t->it_total += it; t->it_solves++;
-    if (t->it_min < 0 || it < t->it_min) t->it_min = it;
-    if (it > t->it_max) t->it_max = it;
-}
-
-// Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres):
-// classical Gram-Schmidt applied twice (fused passes), Givens rotations on the host.
-int gmres(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hipStream_t s) {
+// Defect correction (dc.f:1-151), at most m iterations (slots of them enqueued).
+int dcorr(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, int stage) {
     const size_t n = t->g.n;
-    int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
-    double *V = t->V.p, *w = t->w.p;
-    const int nbl = grid_blocks(n);
-    std::vector<double> H((size_t)(m + 1) * (m + 1), 0.0), cs(m + 1), sn(m + 1), gv(m + 2, 0.0), h(MAXK + 1), h2(MAXK + 1);
-    int matvecs = 0, k = 0;
-    if (launch_matvec(t, x, w, s)) { err() = "matvec dispatch failed"; return -1; }
-    matvecs++;
-    hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w);
-    if (precond(t, w, V, s)) return -1;
-    int fail = 0;
-    double beta = norm2_host(t, V, s, fail);
-    if (fail) return -1;
-    const double tol = t->cfg.reltol * beta + t->cfg.abstol;
-    *res = beta;
-    if (beta <= tol) { *iters = matvecs; record_iters(t, matvecs); return 0; }
-    hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(256), 0, s, n, 1.0 / beta, (const double *)V, 0.0, V);
-    gv[0] = beta;
-    double est = beta;
-    const int nb = red_blocks(n);
-    while (true) {
-        k++;
-        double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
-        if (launch_matvec(t, vk, w, s)) { err() = "matvec dispatch failed"; return -1; }
-        matvecs++;
-        if (precond(t, w, vn, s)) return -1;
-        // classical Gram-Schmidt with the DGKS re-orthogonalisation test (second pass only when the
-        // projection removed more than half of the squared norm)
-        if (launch_mdot(k, true, nb, n, V, vn, t->part.p, s)) return -1;
-        hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
-        HIPCHK(hipMemcpyAsync(h.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const double nrm2 = h[k];
-        double nn = nrm2;
-        for (int i = 0; i < k; i++) nn -= h[i] * h[i];
-        double hn;
-        if (nn > 0.5 * nrm2) {
-            hn = std::sqrt(nn);
-            for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i];
-        } else {
-            if (launch_gs_update_dot(k, nb, n, V, t->hv.p, vn, t->part.p, s)) return -1;
-            hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k + 1), dim3(256), 0, s, nb, k + 1, (const double *)t->part.p, t->hv.p);
-            HIPCHK(hipMemcpyAsync(h2.data(), t->hv.p, (k + 1) * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            double n2 = h2[k];
-            for (int i = 0; i < k; i++) n2 -= h2[i] * h2[i];
-            hn = std::sqrt(std::max(n2, 0.0));
-            for (int i = 0; i < k; i++) H[(size_t)i * m + (k - 1)] = h[i] + h2[i];
-        }
-        H[(size_t)k * m + (k - 1)] = hn;
-        // vn = (vn - V h) / hn  with the h of the last pass (in t->hv)
-        if (launch_gs_update(k, nb, n, V, t->hv.p, -1.0, hn > 0.0 ? 1.0 / hn : 1.0, vn, s)) return -1;
-        for (int i = 0; i < k - 1; i++) {
-            double a = H[(size_t)i * m + (k - 1)], bb = H[(size_t)(i + 1) * m + (k - 1)];
-            H[(size_t)i * m + (k - 1)] = cs[i] * a + sn[i] * bb;
-            H[(size_t)(i + 1) * m + (k - 1)] = cs[i] * bb - sn[i] * a;
-        }
-        double a = H[(size_t)(k - 1) * m + (k - 1)], bb = H[(size_t)k * m + (k - 1)];
-        double rr = std::sqrt(a * a + bb * bb), c = 1.0, sv = 0.0;
-        if (rr > 0.0) { c = a / rr; sv = bb / rr; }
-        cs[k - 1] = c; sn[k - 1] = sv;
-        H[(size_t)(k - 1) * m + (k - 1)] = rr;
-        gv[k] = -sv * gv[k - 1];
-        gv[k - 1] = c * gv[k - 1];
-        est = std::fabs(gv[k]);
-        *res = est;
-        if (!(k < m && !(est <= tol) && matvecs < m)) break;
-        if (hn == 0.0) break;
-        if (k + 1 >= MAXK) break;
-    }
-    int kk = k;
-    while (kk > 0 && H[(size_t)(kk - 1) * m + (kk - 1)] == 0.0) kk--;
-    std::vector<double> y(std::max(kk, 1), 0.0);
-    for (int i = kk - 1; i >= 0; i--) {
-        double sacc = gv[i];
-        for (int q = i + 1; q < kk; q++) sacc -= H[(size_t)i * m + q] * y[q];
-        y[i] = sacc / H[(size_t)i * m + i];
-    }
-    if (kk > 0) {
-        HIPCHK(hipMemcpyAsync(t->hv.p, y.data(), kk * 8, hipMemcpyHostToDevice, s));
-        if (launch_gs_update(kk, nb, n, V, t->hv.p, 1.0, 1.0, x, s)) return -1;
-    }
-    HIPCHK(hipStreamSynchronize(s));
-    *iters = matvecs;
-    record_iters(t, matvecs);
-    return (est <= tol) ? 0 : 1;
-}
-
-// Defect correction (dc.f:1-151).
-int dcorr(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hipStream_t s) {
-    const size_t n = t->g.n;
-    const int nbl = grid_blocks(n);
+    const int nbl = grid_blocks(n), nb = red_blocks(n);
     double *w = t->w.p, *z = t->z.p;
-    int matvecs = 0;
-    double tol = 0.0;
-    while (true) {
-        if (launch_matvec(t, x, w, s)) { err() = "matvec dispatch failed"; return -1; }
-        matvecs++;
-        hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w);
+    KrylovState *S = t->ks.p;
+    const int *done = t->done();
+    hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, s, S);
+    for (int it = 1; it <= slots; it++) {
+        if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+        hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
         if (precond(t, w, z, s)) return -1;
-        int fail = 0;
-        double alpha = norm2_host(t, z, s, fail);
-        if (fail) return -1;
-        if (matvecs == 1) tol = t->cfg.reltol * alpha + t->cfg.abstol;
-        *res = alpha;
-        if (alpha <= tol) { *iters = matvecs; record_iters(t, matvecs); return 0; }
-        hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(256), 0, s, n, 1.0, (const double *)z, 1.0, x);
-        if (matvecs >= t->cfg.gmres_m) { *iters = matvecs; record_iters(t, matvecs); HIPCHK(hipStreamSynchronize(s)); return 1; }
+        if (launch_mdot(1, false, nb, n, z, z, t->part.p, done, s)) return -1;
+        reduce(t, 1, nb, done, s);
+        hipLaunchKernelGGL(k_dc_check, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, it, t->cfg.reltol, t->cfg.abstol);
+        hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(256), 0, s, n, 1.0, (const double *)z, 1.0, x, done);
     }
+    hipLaunchKernelGGL(k_dc_finish, dim3(1), dim3(64), 0, s, S, slots, full_slots(t), stage);
+    return 0;
 }
 
-int solve_dev(ow3d_tank *t, const double *b, double *x, int *iters, double *res, hipStream_t s) {
-    return t->cfg.solver == 0 ? dcorr(t, b, x, iters, res, s) : gmres(t, b, x, iters, res, s);
+// Enqueues one Laplace solve of A x = b (x: initial guess in, solution out); slots < 0: whole budget.
+int solve_dev(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots = -1, int stage = 0) {
+    const int f = full_slots(t);
+    if (slots < 0 || slots > f) slots = f;
+    return t->cfg.solver == 0 ? dcorr(t, b, x, s, slots, stage) : gmres(t, b, x, s, slots, stage);
 }
 
 PlaneArgs plane_args(ow3d_tank *t, double *E, double *P) {
@@ -249,6 +196,10 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     {
         const char *ng = std::getenv("OW3D_NO_GRAPH");
         t->use_graph = !(ng && ng[0] == '1');
+        const char *sp = std::getenv("OW3D_SPECULATE");   // 0: every solve enqueued with its whole budget
+        t->speculate = !(sp && sp[0] == '0');
+        // slots = most recently used + margin (a negative margin forces replays: used by the tests)
+        if (const char *mg = std::getenv("OW3D_SLOT_MARGIN")) t->margin = std::atoi(mg);
     }
     Grid &g = t->g;
     g.nx = c.nx; g.ny = c.ny; g.nz = c.nz; g.gx = 1; g.gy = c.ny > 1 ? 1 : 0;
@@ -266,9 +217,12 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (hxx) std::copy(hxx, hxx + nxy, g.hxx.begin());
     if (hyy) std::copy(hyy, hyy + nxy, g.hyy.begin());
     grid_stencils(g);
-    HIPCHK(hipStreamCreate(&t->s));
+    const long long bytes0 = device_bytes();
+    // non-blocking: set-up copies of a new handle (null stream) do not wait for other handles' work
+    HIPCHK(hipStreamCreateWithFlags(&t->s, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&t->ev0));
     HIPCHK(hipEventCreate(&t->ev1));
+    HIPCHK(hipEventCreateWithFlags(&t->ev_done, hipEventDisableTiming));
     hipStream_t s = t->s;
     auto upd = [&](DevBuf<double> &d, const std::vector<double> &v) -> hipError_t {
         hipError_t e = d.alloc(v.size());
@@ -298,18 +252,34 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
         HIPCHK(b->alloc(nxy));
         HIPCHK(hipMemset(b->p, 0, nxy * 8));
     }
+    for (DevBuf<double> *b : {&t->bE, &t->bP, &t->bW}) HIPCHK(b->alloc(nxy));
+    HIPCHK(t->bphi.alloc(n));
+    HIPCHK(hipHostMalloc((void **)&t->report, 8 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(t->report, 0, 8 * sizeof(int));
+    HIPCHK(hipHostGetDevicePointer((void **)&t->report_dev, t->report, 0));
     for (DevBuf<double> *b : {&t->phi, &t->rhs, &t->last0, &t->last1, &t->w, &t->z, &t->io_a, &t->io_b}) {
         HIPCHK(b->alloc(n));
         HIPCHK(hipMemset(b->p, 0, n * 8));
     }
     const int m = c.gmres_m > 1 ? c.gmres_m : 15;
-    HIPCHK(t->V.alloc((size_t)(m + 1) * n));
     HIPCHK(t->part.alloc((size_t)RED_BLOCKS * MAXK));
     HIPCHK(t->hv.alloc(MAXK));
+    HIPCHK(t->V.alloc((size_t)(m + 1) * n));
+    HIPCHK(hipMemset(t->V.p, 0, (size_t)(m + 1) * n * 8));   // unused basis vectors stay finite
+    {
+        KrylovState k0{};
+        k0.it_min = -1;
+        HIPCHK(t->ks.alloc(1));
+        HIPCHK(hipMemcpy(t->ks.p, &k0, sizeof(k0), hipMemcpyHostToDevice));
+    }
     t->mv.nxg = g.nxg; t->mv.nyg = g.nyg; t->mv.nxy = g.nxy;
     t->mv.sx = t->sx.p; t->mv.sy = t->sy.p; t->mv.sz = t->sz.p; t->mv.zs = t->zs.p; t->mv.cf = t->cf.p;
+    t->mv.skip = nullptr;
     if (c.precond == 3 && mg_setup(t.get())) return -1;
-    HIPCHK(hipDeviceSynchronize());
+    for (int &v : t->slots) v = full_slots(t.get());
+    HIPCHK(hipStreamSynchronize(nullptr));
+    HIPCHK(hipStreamSynchronize(t->s));
+    t->bytes = device_bytes() - bytes0;
     (void)s;
     *out = t.release();
     return 0;
@@ -368,6 +338,7 @@ extern "C" int ow3d_precond_apply(ow3d_tank *t, const double *r, double *u) {
     if (!t || !r || !u) { err() = "null argument"; return -1; }
     const size_t n = t->g.n;
     HIPCHK(hipMemcpyAsync(t->w.p, r, n * 8, hipMemcpyHostToDevice, t->s));
+    hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, t->s, t->ks.p);   // clears the done flag
     if (precond(t, t->w.p, t->io_b.p, t->s)) return -1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(u, t->io_b.p, n * 8, hipMemcpyDeviceToHost, t->s));
@@ -380,15 +351,15 @@ extern "C" int ow3d_solve(ow3d_tank *t, const double *rhs, double *sol, int *ite
     const size_t n = t->g.n;
     HIPCHK(hipMemcpyAsync(t->io_a.p, rhs, n * 8, hipMemcpyHostToDevice, t->s));
     HIPCHK(hipMemcpyAsync(t->io_b.p, sol, n * 8, hipMemcpyHostToDevice, t->s));
-    int it = 0; double r = 0.0;
-    int st = solve_dev(t, t->io_a.p, t->io_b.p, &it, &r, t->s);
-    if (st < 0) return st;
+    if (solve_dev(t, t->io_a.p, t->io_b.p, t->s)) return -1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(sol, t->io_b.p, n * 8, hipMemcpyDeviceToHost, t->s));
+    KrylovState k{};
+    HIPCHK(hipMemcpyAsync(&k, t->ks.p, sizeof(k), hipMemcpyDeviceToHost, t->s));
     HIPCHK(hipStreamSynchronize(t->s));
-    if (iters) *iters = it;
-    if (res) *res = r;
-    return st;
+    if (iters) *iters = k.matvecs;
+    if (res) *res = k.est;
+    return k.status;
 }
 
 extern "C" int ow3d_set_surface(ow3d_tank *t, double *E, double *P, double *Ex, double *Exx, double *Ey, double *Eyy, double *Px,
@@ -457,6 +428,8 @@ extern "C" int ow3d_add_relax_zone(ow3d_tank *t, int axis, int i0, int i1, int j
     if (i0 < 0 || j0 < 0 || i1 >= t->g.nxg || j1 >= t->g.nyg || i1 < i0 || j1 < j0 || (axis != 0 && axis != 1)) {
         err() = "relaxation zone outside the grid"; return -1;
     }
+    HIPCHK(hipStreamSynchronize(t->s));
+    t->drop_step_graphs();   // captured steps hold the old zone arrays
     Zone z{axis, i0, i1, j0, j1, (int)t->gam.size()};
     int len = axis == 0 ? i1 - i0 + 1 : j1 - j0 + 1;
     t->gam.insert(t->gam.end(), gam, gam + len);
@@ -499,56 +472,270 @@ extern "C" int ow3d_get_state(ow3d_tank *t, double *E, double *P, double *W, dou
     return 0;
 }
 
-// One RK4 step (Runge_Kutta_4.f90:20-299) on the device.
-static int rk4_step(ow3d_tank *t, double dt, int &status) {
+static void copy_dev(const double *src, double *dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, src, dst, nullptr);
+}
+
+// Enqueues stage `stage` (1..4) of RK4 step number t->tstep (Runge_Kutta_4.f90:20-299): the surface
+// update, the Laplace solve (`slots` iteration slots) and the vertical velocity.  Stage 1 starts with the
+// step's snapshot (the state a replay starts from), stage 4 ends with the step's report to host memory.
+static int stage_body(ow3d_tank *t, double dt, int stage, int slots) {
     hipStream_t s = t->s;
     const size_t nxy = t->g.nxy, n = t->g.n;
     const int nb2 = (int)((nxy + 255) / 256), nbl = grid_blocks(n);
-    t->tstep++;
-    double *Lnew = (t->tstep - 1) % 2 == 0 ? t->last0.p : t->last1.p;
-    double *Llast = (t->tstep - 1) % 2 == 0 ? t->last1.p : t->last0.p;
-    HIPCHK(hipMemcpyAsync(Lnew, t->phi.p, n * 8, hipMemcpyDeviceToDevice, s));
-    int it = 0; double res = 0.0;
+    // LASTPHI history (Runge_Kutta_4.f90:118-121): Lnew = phi at the start of this step, Llast = phi at
+    // the start of the previous one
+    double *Lnew = t->last0.p, *Llast = t->last1.p;
+    if (stage == 1) {
+        copy_dev(t->E.p, t->bE.p, nxy, s); copy_dev(t->P.p, t->bP.p, nxy, s); copy_dev(t->W.p, t->bW.p, nxy, s);
+        copy_dev(t->phi.p, t->bphi.p, n, s);
+        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, t->ks.p);
+        copy_dev(Lnew, Llast, n, s);
+        copy_dev(t->phi.p, Lnew, n, s);
+    }
+    const double *Es = stage == 1 ? t->E.p : t->Et.p;
+    const double *Wst = stage == 1 ? t->W.p : t->Wt.p;
+    double *Eo = stage == 4 ? t->E.p : t->Et.p;
+    double *Po = stage == 4 ? t->P.p : t->Pt.p;
+    StageArgs a = stage_args(t, stage, dt, Es, Wst, Eo, Po);
+    hipLaunchKernelGGL(k_stage, dim3(nb2), dim3(256), 0, s, a);
+    surface_dev(t, Eo, Po, s);
+    hipLaunchKernelGGL(k_fill_rhs, dim3(grid_blocks(n)), dim3(256), 0, s, t->g.nxg, t->g.nyg, t->g.nzg, t->g.gx, t->g.gy,
+                       (const double *)Po, t->rhs.p);
+    if (stage == 1 && t->cfg.extrapolation == 1)
+        hipLaunchKernelGGL(k_extrap1, dim3(nbl), dim3(256), 0, s, n, (const double *)Lnew, (const double *)Llast, t->phi.p);
+    if (stage == 3 && t->cfg.extrapolation == 1 && t->tstep > 1)
+        hipLaunchKernelGGL(k_extrap3, dim3(nbl), dim3(256), 0, s, n, (const double *)Lnew, (const double *)Llast, t->phi.p);
+    if (solve_dev(t, t->rhs.p, t->phi.p, s, slots, stage - 1)) return -1;
+    wvel_dev(t, t->phi.p, stage == 4 ? t->W.p : t->Wt.p, s);
+    if (stage == 4) hipLaunchKernelGGL(k_step_report, dim3(1), dim3(64), 0, s, (const KrylovState *)t->ks.p, t->report_dev);
+    return 0;
+}
+
+// Enqueues step t->tstep stage by stage.  From the second step after a state change on, a stage's launch
+// sequence depends only on dt, the stage and its slots: it is captured once into a hipGraph per
+// (stage, slots) and replayed.
+static int step_launch(ow3d_tank *t, double dt, const int *slots) {
     for (int stage = 1; stage <= 4; stage++) {
-        const double *Es = stage == 1 ? t->E.p : t->Et.p;
-        const double *Wst = stage == 1 ? t->W.p : t->Wt.p;
-        double *Eo = stage == 4 ? t->E.p : t->Et.p;
-        double *Po = stage == 4 ? t->P.p : t->Pt.p;
-        StageArgs a = stage_args(t, stage, dt, Es, Wst, Eo, Po);
-        hipLaunchKernelGGL(k_stage, dim3(nb2), dim3(256), 0, s, a);
-        surface_dev(t, Eo, Po, s);
-        hipLaunchKernelGGL(k_fill_rhs, dim3(grid_blocks(n)), dim3(256), 0, s, t->g.nxg, t->g.nyg, t->g.nzg, t->g.gx, t->g.gy,
-                           (const double *)Po, t->rhs.p);
-        if (stage == 1 && t->cfg.extrapolation == 1)
-            hipLaunchKernelGGL(k_extrap1, dim3(nbl), dim3(256), 0, s, n, (const double *)Lnew, (const double *)Llast, t->phi.p);
-        if (stage == 3 && t->cfg.extrapolation == 1 && t->tstep > 1)
-            hipLaunchKernelGGL(k_extrap3, dim3(nbl), dim3(256), 0, s, n, (const double *)Lnew, (const double *)Llast, t->phi.p);
-        int st = solve_dev(t, t->rhs.p, t->phi.p, &it, &res, s);
-        if (st < 0) return st;
-        status |= st;
-        wvel_dev(t, t->phi.p, stage == 4 ? t->W.p : t->Wt.p, s);
+        if (!t->use_graph || t->tstep < 2) {
+            if (stage_body(t, dt, stage, slots[stage - 1])) return -1;
+            continue;
+        }
+        if (dt != t->graph_dt) { t->drop_step_graphs(); t->graph_dt = dt; }
+        const std::array<int, 2> key{stage, slots[stage - 1]};
+        auto it = t->graphs.find(key);
+        if (it == t->graphs.end()) {
+            const auto c0 = std::chrono::steady_clock::now();
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(t->s, hipStreamCaptureModeThreadLocal));
+            t->capturing = true;
+            const int rc = stage_body(t, dt, stage, slots[stage - 1]);
+            t->capturing = false;
+            hipError_t e = hipStreamEndCapture(t->s, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return -1; }
+            HIPCHK(e);
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(e);
+            it = t->graphs.emplace(key, exec).first;
+            t->captures++;
+            t->capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+        }
+        HIPCHK(hipGraphLaunch(it->second, t->s));
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Next step: t->tstep advances, enqueued with the current slots.
+static int step_next(ow3d_tank *t, double dt) {
+    t->tstep++;
+    return step_launch(t, dt, t->slots);
+}
+
+// Called once the step in flight has completed.  Rolls it back and re-enqueues it with every solve's
+// whole budget when a solve ran out of slots (*replayed = true; call again when that completes);
+// otherwise sets the slots of the next step from recent use.
+static int step_settle(ow3d_tank *t, double dt, bool *replayed) {
+    *replayed = false;
+    const volatile int *r = t->report;
+    const int F = full_slots(t);
+    if (r[0]) {
+        hipStream_t s = t->s;
+        const size_t nxy = t->g.nxy, n = t->g.n;
+        t->replays++;
+        copy_dev(t->bE.p, t->E.p, nxy, s); copy_dev(t->bP.p, t->P.p, nxy, s); copy_dev(t->bW.p, t->W.p, nxy, s);
+        copy_dev(t->bphi.p, t->phi.p, n, s);
+        copy_dev(t->last1.p, t->last0.p, n, s);   // stage 1 shifts the history again
+        hipLaunchKernelGGL(k_step_restore, dim3(1), dim3(64), 0, s, t->ks.p);
+        surface_dev(t, t->E.p, t->P.p, s);   // plane derivatives and sigma coefficients of the restored surface
+        const int full[4] = {F, F, F, F};
+        for (int &v : t->slots) v = F;
+        *replayed = true;
+        return step_launch(t, dt, full);
+    }
+    if (!t->speculate) return 0;
+    t->hist.push_back({r[1], r[2], r[3], r[4]});
+    if (t->hist.size() > 8) t->hist.erase(t->hist.begin());
+    // grow at once, shrink only by more than one slot (fewer distinct stage graphs)
+    for (int q = 0; q < 4; q++) {
+        int mx = 0;
+        for (const auto &h : t->hist) mx = std::max(mx, h[q]);
+        const int need = std::min(F, std::max(1, mx + t->margin));
+        if (need > t->slots[q] || need + 1 < t->slots[q]) t->slots[q] = need;
+    }
+    return 0;
+}
+
+// Steps enqueued by rk4_steps_enqueue: zero the non-convergence flag of the batch of steps first.
+static int status_reset(ow3d_tank *t) {
+    HIPCHK(hipMemsetAsync(&t->ks.p->status_any, 0, sizeof(int), t->s));
+    return 0;
+}
+
+static int steps_status(ow3d_tank *t) {
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, &t->ks.p->status_any, sizeof(int), hipMemcpyDeviceToHost));
+    return st ? 1 : 0;
+}
+
+// Waits for the step in flight and settles it (replaying as often as needed).
+static int step_finish(ow3d_tank *t, double dt) {
+    bool again = true;
+    while (again) {
+        HIPCHK(hipStreamSynchronize(t->s));
+        if (step_settle(t, dt, &again)) return -1;
+    }
     return 0;
 }
 
 extern "C" int ow3d_rk4_steps(ow3d_tank *t, int nsteps, double time0, double dt) {
     if (!t) return -1;
     (void)time0;
-    int status = 0;
-    for (int q = 0; q < nsteps; q++) {
-        int rc = rk4_step(t, dt, status);
-        if (rc < 0) return rc;
-    }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    if (status_reset(t)) return -1;
+    for (int q = 0; q < nsteps; q++)
+        if (step_next(t, dt) || step_finish(t, dt)) return -1;
     HIPCHK(hipStreamSynchronize(t->s));
-    return status ? 1 : 0;
+    return steps_status(t);
+}
+
+extern "C" int ow3d_rk4_steps_async(ow3d_tank *t, int nsteps, double dt) {
+    if (!t || nsteps < 0) { err() = "null handle or negative step count"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    if (status_reset(t)) return -1;
+    t->async_left = nsteps;
+    t->async_dt = dt;
+    if (nsteps == 0) return 0;
+    if (step_next(t, dt)) return -1;
+    HIPCHK(hipEventRecord(t->ev_done, t->s));
+    t->async_busy = true;
+    return 0;
+}
+
+// Advances the handle's asynchronous steps as far as completed work allows: 1 when all are done.
+extern "C" int ow3d_query(ow3d_tank *t) {
+    if (!t) return -1;
+    if (!t->async_busy) return 1;
+    hipError_t e = hipEventQuery(t->ev_done);
+    if (e == hipErrorNotReady) return 0;
+    HIPCHK(e);
+    bool replayed = false;
+    if (step_settle(t, t->async_dt, &replayed)) return -1;
+    if (!replayed && --t->async_left == 0) { t->async_busy = false; return 1; }
+    if (!replayed && step_next(t, t->async_dt)) return -1;
+    HIPCHK(hipEventRecord(t->ev_done, t->s));
+    return 0;
+}
+
+extern "C" int ow3d_wait(ow3d_tank *t) {
+    if (!t) return -1;
+    while (true) {
+        if (t->async_busy) HIPCHK(hipEventSynchronize(t->ev_done));
+        const int r = ow3d_query(t);
+        if (r < 0) return r;
+        if (r == 1) break;
+    }
+    return steps_status(t);
+}
+
+extern "C" int ow3d_batch_rk4_steps(ow3d_tank *const *tanks, int ntanks, int nsteps, const double *dt) {
+    if (!tanks || !dt || ntanks < 0 || nsteps < 0) { err() = "null argument"; return -1; }
+    for (int i = 0; i < ntanks; i++) {
+        if (!tanks[i]) { err() = "null handle in batch"; return -1; }
+        if (tanks[i]->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    }
+    // every tank advances on its own stream as soon as its previous step has settled (no barrier
+    // between tanks); the host polls the tanks' step events
+    for (int i = 0; i < ntanks; i++)
+        if (ow3d_rk4_steps_async(tanks[i], nsteps, dt[i])) return -1;
+    for (int left = ntanks; left > 0;) {
+        left = 0;
+        bool progressed = false;
+        for (int i = 0; i < ntanks; i++) {
+            if (!tanks[i]->async_busy) continue;
+            const int before = tanks[i]->async_left;
+            const int r = ow3d_query(tanks[i]);
+            if (r < 0) return r;
+            if (r == 0) left++;
+            progressed |= (r == 1) || tanks[i]->async_left != before;
+        }
+        if (left > 0 && !progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    int st = 0;
+    for (int i = 0; i < ntanks; i++) {
+        const int r = steps_status(tanks[i]);
+        if (r < 0) return r;
+        st |= r;
+    }
+    return st;
+}
+
+extern "C" int ow3d_step_stats(const ow3d_tank *t, long long *replays, long long *captures, double *capture_ms) {
+    if (!t) return -1;
+    if (replays) *replays = t->replays;
+    if (captures) *captures = t->captures;
+    if (capture_ms) *capture_ms = t->capture_ms;
+    return 0;
 }
 
 extern "C" int ow3d_stats(const ow3d_tank *t, long long *total, long long *solves, int *itmin, int *itmax) {
     if (!t) return -1;
-    if (total) *total = t->it_total;
-    if (solves) *solves = t->it_solves;
-    if (itmin) *itmin = t->it_min;
-    if (itmax) *itmax = t->it_max;
+    KrylovState k{};
+    HIPCHK(hipStreamSynchronize(t->s));
+    HIPCHK(hipMemcpy(&k, t->ks.p, sizeof(k), hipMemcpyDeviceToHost));
+    if (total) *total = k.it_total;
+    if (solves) *solves = k.it_solves;
+    if (itmin) *itmin = k.it_min;
+    if (itmax) *itmax = k.it_max;
     return 0;
+}
+
+extern "C" long long ow3d_device_bytes(const ow3d_tank *t) { return t ? t->bytes : -1; }
+
+// Device memory ow3d_create would allocate for cfg (the buffers it allocates, without the allocator's
+// rounding): fine-grid vectors and Krylov basis, surface planes, operator tables, multigrid levels.
+extern "C" long long ow3d_estimate_bytes(const ow3d_config *cfg) {
+    if (!cfg || cfg->nx < 2 || cfg->ny < 1 || cfg->nz < 3) { err() = "invalid configuration"; return -1; }
+    Grid g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz; g.gx = 1; g.gy = cfg->ny > 1 ? 1 : 0;
+    g.a = cfg->alpha; g.b = cfg->ny > 1 ? cfg->beta : 0; g.c = cfg->gamma;
+    g.alloc();
+    const long long n = (long long)g.n, nxy = g.nxy;
+    const int m = cfg->gmres_m > 1 ? cfg->gmres_m : 15;
+    long long b = 8 * n * (8 + m + 1 + 1) + 8 * nxy * (25 + 3);   // + step snapshot (phi, E, P, W)
+    b += 8LL * ((long long)g.nxg * 2 * g.ra + (cfg->ny > 1 ? (long long)g.nyg * 2 * g.rb : 6) + 2LL * g.nzg * 2 * g.rc);
+    b += 8LL * RED_BLOCKS * MAXK + 8LL * MAXK + (long long)sizeof(KrylovState);
+    if (cfg->precond == 3) {
+        std::vector<Grid> lev;
+        if (!mg_levels(g, cfg->mg_maxgrids, lev, err())) return -1;
+        const long long p = cfg->mg_precision == 0 ? 8 : 4;
+        for (size_t k = 0; k < lev.size(); k++) {
+            const Grid &L = lev[k];
+            b += p * ((long long)L.n * 4 + 6LL * L.nxy + 12LL * (L.nxg + L.nyg + L.nzg)) + 8LL * (L.nxg + L.nyg + L.nzg);
+            if (k == 0) b += p * (long long)L.n * (long long)L.n;
+        }
+    }
+    return b;
 }

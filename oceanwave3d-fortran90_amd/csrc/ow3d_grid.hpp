@@ -100,11 +100,40 @@ inline int no_grids(int nx, int ny, int nz) {
     return (int)std::lround(tmp);
 }
 
+// Directions halved from an (nx, ny, nz) level: coarsening strategy 0 of MGPreProcess.f90:140-178 (halve
+// x / y while larger than Nz, all three when equal).  Where that would halve an even column count (no
+// nested coarse sigma grid; the reference stops with an error, MGPreProcess.f90:183-211) the column is
+// kept and x / y are halved while they have an odd count above 3.  Returns false when nothing can be halved.
+inline bool coarsen_flags(int nx, int ny, int nz, int &xc, int &yc, int &zc) {
+    xc = yc = zc = 0;
+    if (nx == 1) { if (ny == nz) { yc = 1; zc = 1; } else yc = 1; }
+    else if (ny == 1) { if (nx == nz) { xc = 1; zc = 1; } else xc = 1; }
+    else if (nx == ny && nx == nz) { xc = yc = zc = 1; }
+    else { if (nx > nz) xc = 1; if (ny > nz) yc = 1; if (nx < nz && ny < nz) zc = 1; }
+    if (zc && nz % 2 == 0) {
+        zc = 0;
+        xc = (nx > 3 && nx % 2 == 1) ? 1 : 0;
+        yc = (ny > 3 && ny % 2 == 1) ? 1 : 0;
+    }
+    return xc || yc || zc;
+}
+
 // Builds levels[0] (coarsest) .. levels[L-1] (finest, = fine grid with 2nd-order stencils).
 inline bool mg_levels(const Grid &fine, int maxgrids, std::vector<Grid> &lev, std::string &err) {
     int L = no_grids(fine.nx, fine.ny, fine.nz);
     if (maxgrids > 0 && L > maxgrids) L = maxgrids;
     if (L < 1) L = 1;
+    // coarsening strategy 0 with the even-column rule of coarsen_flags: stop when nothing can be halved
+    {
+        int nx = fine.nx, ny = fine.ny, nz = fine.nz;
+        for (int q = 1; q < L; q++) {
+            int xc, yc, zc;
+            if (!coarsen_flags(nx, ny, nz, xc, yc, zc)) { L = q; break; }
+            if (xc) nx = (nx - 1) / 2 + 1;
+            if (yc) ny = (ny - 1) / 2 + 1;
+            if (zc) nz = (nz - 1) / 2 + 1;
+        }
+    }
     lev.assign(L, Grid());
     for (int k = L - 1; k >= 0; k--) {
         Grid &g = lev[k];
@@ -112,10 +141,7 @@ inline bool mg_levels(const Grid &fine, int maxgrids, std::vector<Grid> &lev, st
         else {
             const Grid &f = lev[k + 1];
             int NFx = f.nx, NFy = f.ny, NFz = f.nz, xc = 0, yc = 0, zc = 0;
-            if (NFx == 1) { if (NFy == NFz) { yc = 1; zc = 1; } else yc = 1; }
-            else if (NFy == 1) { if (NFx == NFz) { xc = 1; zc = 1; } else xc = 1; }
-            else if (NFx == NFy && NFx == NFz) { xc = yc = zc = 1; }
-            else { if (NFx > NFz) xc = 1; if (NFy > NFz) yc = 1; if (NFx < NFz && NFy < NFz) zc = 1; }
+            coarsen_flags(NFx, NFy, NFz, xc, yc, zc);
             if (xc && NFx % 2 == 0) { err = "multigrid needs odd Nx on every coarsened level"; return false; }
             if (yc && NFy % 2 == 0) { err = "multigrid needs odd Ny on every coarsened level"; return false; }
             if (zc && NFz % 2 == 0) { err = "multigrid needs odd Nz on every coarsened level"; return false; }

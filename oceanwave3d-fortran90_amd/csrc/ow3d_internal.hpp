@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,17 +34,29 @@ std::string &err();
         }                                                                                   \
     } while (0)
 
-constexpr int MAXK = 32;     // Krylov basis limit (GMRESmaxiterations <= MAXK - 1)
+constexpr int MAXK = KRYLOV_MAXK;   // Krylov basis limit (GMRESmaxiterations <= MAXK - 1)
 constexpr int RED_BLOCKS = 2048;
+
+// bytes of device memory held by the library's buffers (all handles of the process)
+std::atomic<long long> &device_bytes();
 
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) { (void)hipFree(p); device_bytes() -= (long long)bytes(); }
+        p = nullptr; n = 0;
+    }
+    size_t bytes() const { return (n ? n : 1) * sizeof(T); }
     hipError_t alloc(size_t count) {
+        release();
         n = count;
-        return hipMalloc((void **)&p, (count ? count : 1) * sizeof(T));
+        hipError_t e = hipMalloc((void **)&p, bytes());
+        if (e == hipSuccess) device_bytes() += (long long)bytes();
+        else { p = nullptr; n = 0; }
+        return e;
     }
     hipError_t upload(const T *h, size_t count, hipStream_t s) {
         return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
@@ -93,16 +108,44 @@ struct ow3d_tank {
     bool fused_rb = true;             // one launch per zebra sweep (red and black halves) on the upper levels
     BottomArgs<double> bot64{};
     BottomArgs<float> bot32{};
-    // statistics
-    long long it_total = 0, it_solves = 0;
-    int it_min = -1, it_max = 0;
+    // device-driven solves: scalar state, statistics and the done flag every solve kernel reads
+    DevBuf<KrylovState> ks;
+    const int *done() const { return &ks.p->done; }
+    const int *skip2() const { return &ks.p->skip2; }
     long tstep = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    long long bytes = 0;                // device memory of this handle
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     bool use_graph = true;
+    bool capturing = false;             // a whole RK4 step is being captured (no nested graph launches)
     hipGraphExec_t pg_exec = nullptr;   // captured preconditioner (V cycles)
     const void *mg_out = nullptr;       // finest-level buffer holding the preconditioner result
+    // Speculative RK4 steps: each stage's solve is enqueued with slots[stage] iteration slots (recent
+    // use plus a margin) instead of its whole budget; a step in which a solve used all its slots without
+    // stopping is rolled back from the snapshot (bE, bP, bW, bphi) and replayed with the whole budget.
+    DevBuf<double> bE, bP, bW, bphi;
+    int *report = nullptr, *report_dev = nullptr;   // pinned: overflow, amax[4] of the last step
+    int slots[4] = {0, 0, 0, 0};
+    std::vector<std::array<int, 4>> hist;           // slot use of recent steps
+    long long replays = 0;
+    bool speculate = true;
+    int margin = 2;
+    std::map<std::array<int, 2>, hipGraphExec_t> graphs;   // (stage, slots) -> captured stage
+    double graph_dt = 0.0;
+    long long captures = 0;
+    double capture_ms = 0.0;
+    // asynchronous stepping
+    int async_left = 0;
+    double async_dt = 0.0;
+    bool async_busy = false;
+    void drop_step_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
     ~ow3d_tank() {
+        drop_step_graphs();
+        if (report) (void)hipHostFree(report);
         if (pg_exec) (void)hipGraphExecDestroy(pg_exec);
+        if (ev_done) (void)hipEventDestroy(ev_done);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (s) (void)hipStreamDestroy(s);
@@ -110,8 +153,8 @@ struct ow3d_tank {
 };
 
 namespace ow3d_host {
-// high-order operator (ow3d_matvec.hip)
-int launch_matvec(const ow3d_tank *t, const double *phi, double *y, hipStream_t s);
+// high-order operator (ow3d_matvec.hip); skip: the solve's done flag for launches inside a solve
+int launch_matvec(const ow3d_tank *t, const double *phi, double *y, hipStream_t s, const int *skip = nullptr);
 bool matvec_supported(int nzg, int A);
 // multigrid (ow3d_mg.hip): hierarchy set-up for t->cfg and u = M^-1 r on device vectors
 int mg_setup(ow3d_tank *t);

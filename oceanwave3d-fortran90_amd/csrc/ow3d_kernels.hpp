@@ -17,6 +17,11 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int cclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Kernels launched inside a device-driven Laplace solve take the solve's "done" flag and return at once
+// when it is set, so a fixed launch sequence (a captured graph) costs almost nothing after convergence.
+// nullptr: always run.
+__device__ __forceinline__ bool solve_done(const int *skip) { return skip != nullptr && *skip != 0; }
+
 // ---------------------------------------------------------------------------------------------
 // High-order matvec  (BuildLinearSystem.f90:1-290)
 // ---------------------------------------------------------------------------------------------
@@ -26,6 +31,7 @@ struct MvArgs {
     const double *zs;            // sigma of each level (0 on the ghost level)
     const double *cf;            // 7 planes: sigma coefficients affine in sigma, s2 = A2 + sigma*B2, s4 = A4 + sigma*B4,
                                  // s3 = A3 + sigma*B3, s5 = S5; planes (A2, B2, A4, B4, A3, B3, S5)
+    const int *skip;             // solve_done flag (nullptr: always run)
 };
 
 // Per-column coefficient planes from the surface and the bottom: DetermineTransformationConstantsArray.f90
@@ -81,6 +87,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // derivatives go to LDS); phase 2 the sigma derivatives, the mixed derivatives and the row.
 template <int NZ, int A, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    if (solve_done(g.skip)) return;
     constexpr int TX = 16, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R, NC = TX * TY, NP = NC * NZ;
     constexpr int PER = (NP + NT - 1) / NT;
     __shared__ double sP[WX * WY * NZ];
@@ -243,6 +250,7 @@ __global__ __launch_bounds__(NT) void k_matvec3p(MvArgs g, const double *__restr
 // as coalesced row segments.
 template <int NZ, int A, int TY>
 __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    if (solve_done(g.skip)) return;
     constexpr int TX = 16, NT = TX * TY, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
     constexpr int PS = (WX * WY) | 1, NC = TX * TY;
     __shared__ double sP[NZ * PS];
@@ -425,6 +433,7 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
 // x-z problems (Ny == 1), one thread per grid point: block = 64 columns x NZ levels.
 template <int NZ, int A>
 __global__ __launch_bounds__(256) void k_matvec2p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    if (solve_done(g.skip)) return;
     constexpr int NT = 256, TX = 64, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, NP = TX * NZ, PER = (NP + NT - 1) / NT;
     __shared__ double sP[WX * NZ];
     __shared__ double sDx[NP];
@@ -671,22 +680,27 @@ static __global__ void k_fill_rhs(int nxg, int nyg, int nzg, int gx, int gy, con
 // Vector kernels (grid-stride)
 // ---------------------------------------------------------------------------------------------
 // y = a*x + b*y
-static __global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+static __global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y,
+                               const int *skip = nullptr) {
+    if (solve_done(skip)) return;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = a * x[q] + b * y[q];
 }
 // y = b - y
-static __global__ void k_bminus(size_t n, const double *__restrict__ b, double *__restrict__ y) {
+static __global__ void k_bminus(size_t n, const double *__restrict__ b, double *__restrict__ y, const int *skip) {
+    if (solve_done(skip)) return;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = b[q] - y[q];
 }
 template <typename S, typename D>
-__global__ void k_convert(size_t n, const S *__restrict__ x, D *__restrict__ y) {
+__global__ void k_convert(size_t n, const S *__restrict__ x, D *__restrict__ y, const int *skip) {
+    if (solve_done(skip)) return;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = (D)x[q];
 }
 template <typename T>
-__global__ void k_fill(size_t n, T v, T *__restrict__ y) {
+__global__ void k_fill(size_t n, T v, T *__restrict__ y, const int *skip) {
+    if (solve_done(skip)) return;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = v;
 }
@@ -717,7 +731,8 @@ __device__ __forceinline__ void block_reduce_store(const double (&acc)[NA], doub
 // part[b][i] = sum over block b's range of V_i . w for i < K (V_i = V + i*n); NORM adds w . w at index K.
 template <int K, int NORM, int PSTRIDE>
 __global__ __launch_bounds__(256) void k_mdot(size_t n, const double *__restrict__ V, const double *__restrict__ w,
-                                               double *__restrict__ part) {
+                                               double *__restrict__ part, const int *skip) {
+    if (solve_done(skip)) return;
     double acc[K + NORM];
 #pragma unroll
     for (int i = 0; i < K + NORM; i++) acc[i] = 0.0;
@@ -736,7 +751,8 @@ __global__ __launch_bounds__(256) void k_mdot(size_t n, const double *__restrict
 // v <- v - sum_i h_i V_i; part gets the dots of the new v with V_i (i < K) and its squared norm (index K).
 template <int K, int PSTRIDE>
 __global__ __launch_bounds__(256) void k_gs_update_dot(size_t n, const double *__restrict__ V, const double *__restrict__ h,
-                                                        double *__restrict__ v, double *__restrict__ part) {
+                                                        double *__restrict__ v, double *__restrict__ part, const int *skip) {
+    if (solve_done(skip)) return;
     double acc[K + 1], hh[K];
 #pragma unroll
     for (int i = 0; i < K; i++) { acc[i] = 0.0; hh[i] = h[i]; }
@@ -756,10 +772,14 @@ __global__ __launch_bounds__(256) void k_gs_update_dot(size_t n, const double *_
     block_reduce_store<K + 1, PSTRIDE>(acc, part);
 }
 
-// v <- (v + sign * sum_i h_i V_i) * scale  (sign -1: orthogonalisation; +1, scale 1: solution update).
+// v <- (v + sign * sum_i h_i V_i) * scale  (sign -1: orthogonalisation; +1, scale 1: solution update);
+// scale_dev, when given, holds the scale in device memory.
 template <int K>
 __global__ __launch_bounds__(256) void k_gs_update(size_t n, const double *__restrict__ V, const double *__restrict__ h,
-                                                    double sign, double scale, double *__restrict__ v) {
+                                                    double sign, double scale, const double *scale_dev, double *__restrict__ v,
+                                                    const int *skip) {
+    if (solve_done(skip)) return;
+    if (scale_dev) scale = *scale_dev;
     double hh[K];
 #pragma unroll
     for (int i = 0; i < K; i++) hh[i] = sign * h[i];
@@ -776,7 +796,8 @@ __global__ __launch_bounds__(256) void k_gs_update(size_t n, const double *__res
 
 // out[i] = sum_b part[b][i]; one block per entry, fixed summation order (deterministic).
 template <int MAXK>
-__global__ __launch_bounds__(256) void k_reduce_parts(int nblk, int k, const double *part, double *out) {
+__global__ __launch_bounds__(256) void k_reduce_parts(int nblk, int k, const double *part, double *out, const int *skip) {
+    if (solve_done(skip)) return;
     const int i = blockIdx.x;
     if (i >= k) return;
     double s = 0.0;
@@ -790,6 +811,199 @@ __global__ __launch_bounds__(256) void k_reduce_parts(int nblk, int k, const dou
 }
 
 // ---------------------------------------------------------------------------------------------
+// Device-driven Krylov solves: the scalar work of GMRES(m) (Hessenberg column, Givens rotations,
+// stopping test, back substitution) and of defect correction runs in one-thread kernels on the solve's
+// stream, so a whole solve is a fixed launch sequence with no host round trip.  After convergence the
+// solve's kernels see `done` and return at once.
+// ---------------------------------------------------------------------------------------------
+constexpr int KRYLOV_MAXK = 32;   // basis limit: GMRESmaxiterations <= KRYLOV_MAXK - 1
+
+struct KrylovState {
+    int done;       // solve finished (converged, breakdown or out of iterations)
+    int skip2;      // 0 only between the two passes of an Arnoldi step that needs the DGKS second pass
+    int k;          // Arnoldi steps taken
+    int kk;         // basis vectors combined into the solution
+    int matvecs;    // operator applications of this solve
+    int status;     // 0 converged, 1 not converged
+    double beta, tol, est, scale;
+    double H[KRYLOV_MAXK * KRYLOV_MAXK];   // Hessenberg entry (i, j) at H[i * KRYLOV_MAXK + j]
+    double cs[KRYLOV_MAXK], sn[KRYLOV_MAXK], g[KRYLOV_MAXK + 1], h1[KRYLOV_MAXK], y[KRYLOV_MAXK];
+    // over all solves of the handle
+    long long it_total, it_solves;
+    int it_min, it_max, status_any;
+    // speculative RK4 steps: a solve enqueued with fewer iteration slots than its budget that used them all
+    // without stopping sets overflow; amax[stage] = most slots a solve of that stage used in this step
+    int overflow, amax[4];
+    long long sv_total, sv_solves;   // statistics at the start of the step (restored when it is replayed)
+    int sv_min, sv_max, sv_status;
+};
+
+// Start of a speculative RK4 step: clear the per-step report, keep the statistics for a replay.
+static __global__ void k_step_begin(KrylovState *S) {
+    if (threadIdx.x != 0) return;
+    S->overflow = 0;
+    for (int i = 0; i < 4; i++) S->amax[i] = 0;
+    S->sv_total = S->it_total; S->sv_solves = S->it_solves; S->sv_min = S->it_min; S->sv_max = S->it_max;
+    S->sv_status = S->status_any;
+}
+// Replay of a step: statistics as they were before it.
+static __global__ void k_step_restore(KrylovState *S) {
+    if (threadIdx.x != 0) return;
+    S->it_total = S->sv_total; S->it_solves = S->sv_solves; S->it_min = S->sv_min; S->it_max = S->sv_max;
+    S->status_any = S->sv_status;
+}
+// End of a step: overflow flag and per-stage slot use to host-visible memory.
+static __global__ void k_step_report(const KrylovState *S, int *out) {
+    if (threadIdx.x != 0) return;
+    out[0] = S->overflow;
+    for (int i = 0; i < 4; i++) out[1 + i] = S->amax[i];
+}
+
+// Start of a solve (before its first operator application).
+static __global__ void k_solve_begin(KrylovState *S) {
+    if (threadIdx.x != 0) return;
+    S->done = 0; S->skip2 = 1; S->k = 0; S->kk = 0; S->matvecs = 1; S->status = 1;
+}
+
+// GMRES: hv[0] = |M^-1 (b - A x0)|^2.  Sets the stopping tolerance (rtol * beta + atol) and the scale of
+// the first basis vector.
+static __global__ void k_gmres_start(KrylovState *S, const double *hv, double rtol, double atol) {
+    if (threadIdx.x != 0) return;
+    const double beta = sqrt(hv[0]);
+    S->beta = beta; S->tol = rtol * beta + atol; S->est = beta;
+    if (beta <= S->tol) { S->done = 1; S->status = 0; return; }
+    S->scale = 1.0 / beta;
+    S->g[0] = beta;
+}
+
+// Hessenberg column k (1-based step) from its projections col[0..k-1] and the new vector's norm hn;
+// previous rotations, the new rotation, the residual estimate and the stopping test.
+__device__ inline void gmres_column(KrylovState *S, int k, const double *col, double hn, int m) {
+    constexpr int M = KRYLOV_MAXK;
+    double *H = S->H;
+    for (int i = 0; i < k; i++) H[i * M + (k - 1)] = col[i];
+    H[k * M + (k - 1)] = hn;
+    for (int i = 0; i < k - 1; i++) {
+        const double a = H[i * M + (k - 1)], b = H[(i + 1) * M + (k - 1)];
+        H[i * M + (k - 1)] = S->cs[i] * a + S->sn[i] * b;
+        H[(i + 1) * M + (k - 1)] = S->cs[i] * b - S->sn[i] * a;
+    }
+    const double a = H[(k - 1) * M + (k - 1)], b = H[k * M + (k - 1)];
+    const double rr = sqrt(a * a + b * b);
+    double c = 1.0, sv = 0.0;
+    if (rr > 0.0) { c = a / rr; sv = b / rr; }
+    S->cs[k - 1] = c; S->sn[k - 1] = sv;
+    H[(k - 1) * M + (k - 1)] = rr;
+    S->g[k] = -sv * S->g[k - 1];
+    S->g[k - 1] = c * S->g[k - 1];
+    const double est = fabs(S->g[k]);
+    S->est = est; S->k = k;
+    S->scale = hn > 0.0 ? 1.0 / hn : 1.0;
+    if (!(k < m && !(est <= S->tol) && S->matvecs < m) || hn == 0.0 || k + 1 >= M) S->done = 1;
+}
+
+// Arnoldi step k, first pass: hv[0..k-1] = V_i . w, hv[k] = w . w (classical Gram-Schmidt).  Keeps the
+// column when the projection removed at most half of the squared norm, else asks for the second pass.
+static __global__ void k_gmres_pass1(KrylovState *S, const double *hv, int k, int m) {
+    if (threadIdx.x != 0 || S->done) return;
+    S->matvecs += 1;
+    const double nrm2 = hv[k];
+    double nn = nrm2;
+    for (int i = 0; i < k; i++) nn -= hv[i] * hv[i];
+    if (nn > 0.5 * nrm2) {
+        gmres_column(S, k, hv, sqrt(nn), m);
+        S->skip2 = 1;
+    } else {
+        for (int i = 0; i < k; i++) S->h1[i] = hv[i];
+        S->skip2 = 0;
+    }
+}
+
+// Arnoldi step k, second (DGKS) pass: hv holds the projections of the once-orthogonalised vector.
+static __global__ void k_gmres_pass2(KrylovState *S, const double *hv, int k, int m) {
+    if (threadIdx.x != 0 || S->skip2) return;
+    double n2 = hv[k], col[KRYLOV_MAXK];
+    for (int i = 0; i < k; i++) { n2 -= hv[i] * hv[i]; col[i] = S->h1[i] + hv[i]; }
+    gmres_column(S, k, col, sqrt(fmax(n2, 0.0)), m);
+    S->skip2 = 1;
+}
+
+__device__ inline void solve_stats(KrylovState *S) {
+    S->it_total += S->matvecs; S->it_solves += 1;
+    if (S->it_min < 0 || S->matvecs < S->it_min) S->it_min = S->matvecs;
+    if (S->matvecs > S->it_max) S->it_max = S->matvecs;
+    S->status_any |= S->status;
+}
+
+// End of a GMRES solve: back substitution H y = g on the kk leading columns (trailing zero pivots
+// dropped), y to S->y; the solution update follows in k_basis_combine.  slots < full: the solve was
+// enqueued with fewer Arnoldi steps than its budget (flags overflow when it used them all).
+static __global__ void k_gmres_finish(KrylovState *S, int slots, int full, int stage) {
+    if (threadIdx.x != 0) return;
+    if (!S->done && slots < full) S->overflow = 1;
+    if (S->k > S->amax[stage]) S->amax[stage] = S->k;
+    constexpr int M = KRYLOV_MAXK;
+    int kk = 0;
+    if (!(S->beta <= S->tol)) {
+        kk = S->k;
+        while (kk > 0 && S->H[(kk - 1) * M + (kk - 1)] == 0.0) kk--;
+        for (int i = kk - 1; i >= 0; i--) {
+            double acc = S->g[i];
+            for (int q = i + 1; q < kk; q++) acc -= S->H[i * M + q] * S->y[q];
+            S->y[i] = acc / S->H[i * M + i];
+        }
+        S->status = (S->est <= S->tol) ? 0 : 1;
+    }
+    S->kk = kk;
+    solve_stats(S);
+}
+
+// x += sum_{i < S->kk} S->y[i] V_i
+static __global__ __launch_bounds__(256) void k_basis_combine(size_t n, const double *__restrict__ V, const KrylovState *S,
+                                                              double *__restrict__ x) {
+    const int kk = S->kk;
+    if (kk == 0) return;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double acc = x[q];
+        for (int i0 = 0; i0 < kk; i0 += 8) {   // eight basis loads in flight per thread
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = (i0 + u < kk) ? V[(size_t)(i0 + u) * n + q] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (i0 + u < kk) acc += S->y[i0 + u] * v[u];
+        }
+        x[q] = acc;
+    }
+}
+
+// v *= S->scale
+static __global__ void k_scale_state(size_t n, const KrylovState *S, double *__restrict__ v) {
+    if (S->done) return;
+    const double a = S->scale;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) v[q] *= a;
+}
+
+// Defect correction, iteration it (1-based): hv[0] = |z|^2 of the preconditioned defect z; the tolerance
+// is set from the first one (dc.f:77-101).
+static __global__ void k_dc_check(KrylovState *S, const double *hv, int it, double rtol, double atol) {
+    if (threadIdx.x != 0 || S->done) return;
+    const double alpha = sqrt(hv[0]);
+    if (it == 1) { S->beta = alpha; S->tol = rtol * alpha + atol; }
+    S->est = alpha; S->matvecs = it;
+    if (alpha <= S->tol) { S->done = 1; S->status = 0; }
+}
+
+static __global__ void k_dc_finish(KrylovState *S, int slots, int full, int stage) {
+    if (threadIdx.x != 0) return;
+    if (!S->done && slots < full) S->overflow = 1;
+    if (S->matvecs > S->amax[stage]) S->amax[stage] = S->matvecs;
+    S->status = S->done ? S->status : 1;
+    S->kk = 0;
+    solve_stats(S);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Multigrid (low-order linearised operator, matrix-free; BuildLinearSystemMatrix.f90 row semantics)
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -798,6 +1012,7 @@ struct LvArgs {
     const T *sx, *sy, *sz;  // 3-point stencil rows [N][2][3]
     const T *zs;            // sigma of each level (0 on the ghost level)
     const T *col;           // 6 planes: 1/h, h_x, h_y, h_x/h, h_y/h, -2(h_x^2+h_y^2)/h^2 + (h_xx+h_yy)/h
+    const int *skip;        // solve_done flag (nullptr: always run)
 };
 
 // Level coefficients as the column kernels read them: 3-point stencil rows sx(i), sy(j), sz(k) (first then
@@ -1383,6 +1598,7 @@ __device__ __forceinline__ bool colour_column(const LvArgs<T> &L, int colour, in
 // Zebra sweep over the columns of one colour, one thread per column, x read and written in HBM.
 template <typename T, int NZ>
 __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    if (solve_done(L.skip)) return;
     int i, j;
     if (!colour_column(L, colour, (int)(blockIdx.x * blockDim.x + threadIdx.x), i, j)) return;
     const int nxg = L.nxg;
@@ -1398,6 +1614,7 @@ __global__ __launch_bounds__(256) void k_mg_zebra(LvArgs<T> L, int colour, T *__
 // are rewritten unchanged; nothing else modifies them during the sweep).
 template <typename T, int NZ>
 __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *__restrict__ x, const T *__restrict__ b) {
+    if (solve_done(L.skip)) return;
     constexpr int TX = 32, TY = 8, WX = TX + 2, WY = TY + 2, NZP = NZ | 1;
     __shared__ T sX[WX * WY * NZP];
     __shared__ T sB[TX * TY * NZP];
@@ -1438,6 +1655,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // (ghost pass, red, black, ghost pass) with x read once and written once.
 template <typename T, int NZ, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
+    if (solve_done(L.skip)) return;
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
     __shared__ T sX[WX * WY * NZP];
@@ -1574,6 +1792,7 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
 
 template <typename T>
 __global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict__ b) {
+    if (solve_done(L.skip)) return;
     ghost_point(L, (int)(blockIdx.x * blockDim.x + threadIdx.x), x, b);
 }
 
@@ -1581,6 +1800,7 @@ __global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict
 // problems: one row of 16 * TY columns), level coefficients staged in LDS with the tile.
 template <typename T, int NZ, int TY>
 __global__ __launch_bounds__(16 * TY) void k_mg_residual_t(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
+    if (solve_done(L.skip)) return;
     constexpr int TX = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1, NT = TX * TY;
     constexpr int RS2 = TX * TY + 2 * R;       // x-z problems: the tile is one row of TX * TY columns
     constexpr int LX = WX * WY > RS2 ? WX * WY : RS2;
@@ -1632,6 +1852,7 @@ struct RestrictGeo {
     int nxgc, nygc, nzgc, nxc, nyc, nzc;             // coarse
     int xc, yc, zc;
     const double *xf, *yf, *zf;                      // fine coordinates
+    const int *skip;                                 // solve_done flag (nullptr: always run)
 };
 // Coarse point q; also clears the coarse iterate xz (the coarse V cycle starts from zero).
 template <typename T>
@@ -1674,6 +1895,7 @@ __device__ __forceinline__ void restrict_point(const RestrictGeo &G, size_t q, c
 }
 template <typename T>
 __global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
+    if (solve_done(G.skip)) return;
     restrict_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, rf, rc, xz);
 }
 
@@ -1722,12 +1944,15 @@ __device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, co
 }
 template <typename T>
 __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__restrict__ xf) {
+    if (solve_done(G.skip)) return;
     prolong_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, ec, xf);
 }
 
 // Coarsest level: x = Minv b with the dense inverse computed at set-up.
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, const T *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x) {
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, const T *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x,
+                                                   const int *skip) {
+    if (solve_done(skip)) return;
     // one 64-lane wave per row: coalesced reads of the row, fixed-order lane sums and wave reduction
     const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n) return;
@@ -1756,6 +1981,7 @@ struct BottomArgs {
     T *b[MG_BOTTOM_MAX];
     T *r[MG_BOTTOM_MAX];
     RestrictGeo G[MG_BOTTOM_MAX];   // G[l]: level l (fine) -> level l-1 (coarse), l >= 1
+    const int *skip;                // solve_done flag (nullptr: always run)
 };
 
 template <typename T, int NZ>
@@ -1807,6 +2033,7 @@ __device__ void bottom_smooth(const LvArgs<T> &L, int nu, T *x, const T *b) {
 // All levels 1 .. nlev-1 of the bottom group have NZ sigma levels; level 0 is solved densely.
 template <typename T, int NZ>
 __global__ __launch_bounds__(512) void k_mg_bottom(BottomArgs<T> A) {
+    if (solve_done(A.skip)) return;
     const int top = A.nlev - 1;
     for (int l = top; l >= 1; l--) {
         bottom_smooth<T, NZ>(A.L[l], A.nu1, A.x[l], A.b[l]);

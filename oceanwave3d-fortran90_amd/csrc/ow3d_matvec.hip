@@ -1,4 +1,4 @@
-// Dispatch of the high-order sigma-Laplace matvec kernels (k_matvec3p / k_matvec2p) over the
+// Dispatch of the high-order sigma-Laplace matvec kernels (k_matvec3c / k_matvec3p / k_matvec2p) over the
 // column depth and the stencil half width.
 #include "ow3d_internal.hpp"
 
@@ -21,56 +21,58 @@ constexpr size_t mv3c_lds() {
     return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA + 7 * 16 * TY) * 8;
 }
 template <int NZ, int A>
-int launch_mv3c(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+int launch_mv3c(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     constexpr int TY = mv3c_lds<NZ, A, 16>() <= 65536 ? 16 : 8;
     static_assert(mv3c_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
     const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, t->mv, phi, y);
+    hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, a, phi, y);
     return 0;
 }
 template <int NZ, int A>
-int launch_mv3(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+int launch_mv3(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     static const bool col = [] { const char *e = std::getenv("OW3D_MV"); return !(e && e[0] == 'p'); }();
-    if (col) return launch_mv3c<NZ, A>(t, phi, y, s);
+    if (col) return launch_mv3c<NZ, A>(t, a, phi, y, s);
     constexpr int TY = mv3_lds<NZ, A, 8>() <= 65536 ? 8 : 4;
     static_assert(mv3_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
     const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), dim3(nb), dim3(256), 0, s, t->mv, phi, y);
+    hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), dim3(nb), dim3(256), 0, s, a, phi, y);
     return 0;
 }
 template <int NZ, int A>
-int launch_mv2(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+int launch_mv2(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     dim3 grid((t->g.nxg + 63) / 64);
-    hipLaunchKernelGGL((k_matvec2p<NZ, A>), grid, dim3(256), 0, s, t->mv, phi, y);
+    hipLaunchKernelGGL((k_matvec2p<NZ, A>), grid, dim3(256), 0, s, a, phi, y);
     return 0;
 }
 
 template <int NZ>
-int dispatch_mv_a(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+int dispatch_mv_a(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     const bool d3 = t->g.ny > 1;
     switch (t->A) {
-        case 1: return d3 ? launch_mv3<NZ, 1>(t, phi, y, s) : launch_mv2<NZ, 1>(t, phi, y, s);
-        case 2: return d3 ? launch_mv3<NZ, 2>(t, phi, y, s) : launch_mv2<NZ, 2>(t, phi, y, s);
-        case 3: return d3 ? launch_mv3<NZ, 3>(t, phi, y, s) : launch_mv2<NZ, 3>(t, phi, y, s);
-        case 4: return d3 ? launch_mv3<NZ, 4>(t, phi, y, s) : launch_mv2<NZ, 4>(t, phi, y, s);
+        case 1: return d3 ? launch_mv3<NZ, 1>(t, a, phi, y, s) : launch_mv2<NZ, 1>(t, a, phi, y, s);
+        case 2: return d3 ? launch_mv3<NZ, 2>(t, a, phi, y, s) : launch_mv2<NZ, 2>(t, a, phi, y, s);
+        case 3: return d3 ? launch_mv3<NZ, 3>(t, a, phi, y, s) : launch_mv2<NZ, 3>(t, a, phi, y, s);
+        case 4: return d3 ? launch_mv3<NZ, 4>(t, a, phi, y, s) : launch_mv2<NZ, 4>(t, a, phi, y, s);
     }
     return -1;
 }
 
 }  // namespace
 
-int launch_matvec(const ow3d_tank *t, const double *phi, double *y, hipStream_t s) {
+int launch_matvec(const ow3d_tank *t, const double *phi, double *y, hipStream_t s, const int *skip) {
+    MvArgs a = t->mv;
+    a.skip = skip;
     switch (t->g.nzg) {
-        case 5: return dispatch_mv_a<5>(t, phi, y, s);
-        case 6: return dispatch_mv_a<6>(t, phi, y, s);
-        case 7: return dispatch_mv_a<7>(t, phi, y, s);
-        case 8: return dispatch_mv_a<8>(t, phi, y, s);
-        case 9: return dispatch_mv_a<9>(t, phi, y, s);
-        case 10: return dispatch_mv_a<10>(t, phi, y, s);
-        case 11: return dispatch_mv_a<11>(t, phi, y, s);
-        case 12: return dispatch_mv_a<12>(t, phi, y, s);
-        case 13: return dispatch_mv_a<13>(t, phi, y, s);
-        case 14: return dispatch_mv_a<14>(t, phi, y, s);
+        case 5: return dispatch_mv_a<5>(t, a, phi, y, s);
+        case 6: return dispatch_mv_a<6>(t, a, phi, y, s);
+        case 7: return dispatch_mv_a<7>(t, a, phi, y, s);
+        case 8: return dispatch_mv_a<8>(t, a, phi, y, s);
+        case 9: return dispatch_mv_a<9>(t, a, phi, y, s);
+        case 10: return dispatch_mv_a<10>(t, a, phi, y, s);
+        case 11: return dispatch_mv_a<11>(t, a, phi, y, s);
+        case 12: return dispatch_mv_a<12>(t, a, phi, y, s);
+        case 13: return dispatch_mv_a<13>(t, a, phi, y, s);
+        case 14: return dispatch_mv_a<14>(t, a, phi, y, s);
     }
     return -1;
 }

@@ -33,15 +33,16 @@ int mg_setup(ow3d_tank *t) {
     return 0;
 }
 
-// u = M^-1 r (device vectors, FP64 in / out).  When r is the handle's residual buffer the V cycle is
-// replayed from a hipGraph captured on first use (one graph per handle; fixed buffers).
+// u = M^-1 r (device vectors, FP64 in / out), skipped once the solve's done flag is set.  When r is the
+// handle's residual buffer the V cycle is replayed from a hipGraph captured on first use (one graph per
+// handle; fixed buffers), unless a whole RK4 step is being captured (its graph then holds the V cycle).
 int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
     const size_t n = t->g.n;
     if (t->cfg.precond != 3) {
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, u);
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, u, t->done());
         return 0;
     }
-    if (t->use_graph && r == t->w.p && s == t->s) {
+    if (t->use_graph && !t->capturing && r == t->w.p && s == t->s) {
         if (!t->pg_exec) {
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -57,9 +58,9 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
         if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s))) return -1;
     }
     if (t->cfg.mg_precision == 0)
-        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg_out, u);
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg_out, u, t->done());
     else
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg_out, u);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const float *)t->mg_out, u, t->done());
     return 0;
 }
 

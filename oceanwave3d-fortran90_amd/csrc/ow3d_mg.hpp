@@ -197,18 +197,20 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         L->args.nxg = g.nxg; L->args.nyg = g.nyg; L->args.nzg = g.nzg; L->args.nxy = g.nxy;
         L->args.gx = g.gx; L->args.gy = g.gy; L->args.dim3 = g.ny > 1;
         L->args.sx = L->sx.p; L->args.sy = L->sy.p; L->args.sz = L->sz.p; L->args.zs = L->zs.p; L->args.col = L->col.p;
+        L->args.skip = t->done();
         mg.push_back(std::move(L));
     }
     (void)s;
     return 0;
 }
 
-inline RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const double *yf, const double *zf) {
+inline RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const double *yf, const double *zf, const int *skip) {
     RestrictGeo G{};
     G.nxgf = f.nxg; G.nygf = f.nyg; G.nzgf = f.nzg; G.nxf = f.nx; G.nyf = f.ny; G.nzf = f.nz; G.gx = f.gx; G.gy = f.gy;
     G.nxgc = c.nxg; G.nygc = c.nyg; G.nzgc = c.nzg; G.nxc = c.nx; G.nyc = c.ny; G.nzc = c.nz;
     G.xc = c.xc; G.yc = c.yc; G.zc = c.zc;
     G.xf = xf; G.yf = yf; G.zf = zf;
+    G.skip = skip;
     return G;
 }
 
@@ -223,8 +225,9 @@ void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, Bot
     for (int k = 0; k <= t->bot_top; k++) {
         MgLevel<T> &L = *mg[k];
         A.L[k] = L.args; A.x[k] = L.x.p; A.b[k] = L.b.p; A.r[k] = L.r.p;
-        if (k >= 1) A.G[k] = geo(L.g, mg[k - 1]->g, L.xf.p, L.yf.p, L.zf.p);
+        if (k >= 1) A.G[k] = geo(L.g, mg[k - 1]->g, L.xf.p, L.yf.p, L.zf.p, t->done());
     }
+    A.skip = t->done();
 }
 
 template <typename T>
@@ -304,14 +307,14 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
         return 0;
     }
     if (k == 0) {
-        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p);
+        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p, t->done());
         cur = L.x.p;
         return 0;
     }
     MgLevel<T> &C = *mg[k - 1];
     if (smooth(t, L, t->cfg.nu1, cur, s)) return -1;
     if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
-    RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p);
+    RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p, t->done());
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
@@ -328,8 +331,8 @@ int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, con
     const size_t n = t->g.n;
     const int top = t->nlev - 1;
     MgLevel<T> &F = *mg[top];
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p);
-    hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, T(0), F.x.p);
+    hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, t->done());
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, T(0), F.x.p, t->done());
     T *cur = F.x.p;
     for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
         if (t->nlev == 1 && it > 0) break;

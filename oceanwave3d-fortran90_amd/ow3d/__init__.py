@@ -11,6 +11,10 @@ The entry points mirror the reference subroutines they replace (see include/ow3d
     Tank.rhs_free_surface   rhsFreeSurface3D / rhsLinearFreeSurface3D
     Tank.rk4_steps       Runge_Kutta_4 driven by OceanWave3DTakeATimeStep
 
+and, with no counterpart in the reference, many tanks at once: batch_rk4_steps (tanks advanced
+concurrently, each RK4 step one hipGraph replay on the tank's stream), Tank.rk4_steps_async /
+query / wait for request loops (ow3d/serve.py), and the device-memory figures used for admission.
+
 Arrays follow the reference layout: 3D fields have shape (nyg, nxg, nzg) (sigma fastest),
 plane fields (nyg, nxg).  The library must be built (``python build.py``); nothing here falls
 back to a CPU implementation.
@@ -48,7 +52,8 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_precond_apply", "ow3d_solve", "ow3d_set_surface", "ow3d_vertical_velocity",
            "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
-           "ow3d_time_matvec"]
+           "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
+           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats"]
 
 
 class OW3DError(RuntimeError):
@@ -86,6 +91,15 @@ def lib():
         L.ow3d_stream.restype = ctypes.c_void_p
         L.ow3d_stream.argtypes = [P]
         L.ow3d_time_matvec.argtypes = [P, I, D]
+        L.ow3d_batch_rk4_steps.argtypes = [ctypes.POINTER(P), I, I, D]
+        L.ow3d_rk4_steps_async.argtypes = [P, I, ctypes.c_double]
+        L.ow3d_query.argtypes = [P]
+        L.ow3d_wait.argtypes = [P]
+        L.ow3d_device_bytes.restype = ctypes.c_longlong
+        L.ow3d_device_bytes.argtypes = [P]
+        L.ow3d_step_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), D]
+        L.ow3d_estimate_bytes.restype = ctypes.c_longlong
+        L.ow3d_estimate_bytes.argtypes = [ctypes.POINTER(Config)]
         _lib = L
     return _lib
 
@@ -204,3 +218,47 @@ class Tank:
         ms = ctypes.c_double(0.0)
         _check(lib().ow3d_time_matvec(self.h, int(n_launches), ctypes.pointer(ms)), "ow3d_time_matvec")
         return ms.value
+
+    def rk4_steps_async(self, nsteps, dt):
+        """Enqueues nsteps RK4 steps on the tank's stream and returns at once."""
+        _check(lib().ow3d_rk4_steps_async(self.h, int(nsteps), float(dt)), "ow3d_rk4_steps_async")
+
+    def query(self):
+        """True once the steps enqueued by rk4_steps_async are done."""
+        return _check(lib().ow3d_query(self.h), "ow3d_query") == 1
+
+    def wait(self):
+        """Blocks until the enqueued steps are done; 1 if a solve missed its tolerance."""
+        return _check(lib().ow3d_wait(self.h), "ow3d_wait")
+
+    def device_bytes(self):
+        return int(lib().ow3d_device_bytes(self.h))
+
+    def step_stats(self):
+        """(replays, captures, capture_ms): speculative steps rolled back and replayed with the whole
+        iteration budget, stage graphs captured, host milliseconds spent capturing."""
+        r = ctypes.c_longlong(); c = ctypes.c_longlong(); ms = ctypes.c_double()
+        _check(lib().ow3d_step_stats(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.pointer(ms)), "ow3d_step_stats")
+        return r.value, c.value, ms.value
+
+    def replays(self):
+        return self.step_stats()[0]
+
+
+def estimate_bytes(cfg):
+    """Device memory Tank(cfg, ...) would allocate (before the allocator's rounding)."""
+    c = dict(DEFAULTS)
+    c.update(cfg)
+    v = lib().ow3d_estimate_bytes(ctypes.byref(Config(**c)))
+    if v < 0:
+        raise OW3DError(f"ow3d_estimate_bytes: {lib().ow3d_last_error().decode()}")
+    return int(v)
+
+
+def batch_rk4_steps(tanks, nsteps, dts):
+    """Advances every tank by nsteps RK4 steps (tank i with time step dts[i]) concurrently on the device;
+    returns 1 if any solve missed its tolerance."""
+    n = len(tanks)
+    hs = (ctypes.c_void_p * n)(*[t.h for t in tanks])
+    d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (n,)))
+    return _check(lib().ow3d_batch_rk4_steps(hs, n, int(nsteps), _p(d)), "ow3d_batch_rk4_steps")

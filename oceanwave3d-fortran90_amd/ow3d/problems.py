@@ -162,3 +162,55 @@ def workload(name, seed=0):
         cfg = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly)
         return cfg, bath, E, P, dt, sponge_zones(nx, ny, lx, ly, 5 * lam)
     raise KeyError(name)
+
+
+BATCH_SIZES = (65, 129, 257, 513)
+BATCH_NZ = (6, 9, 12)
+BATCH_ORDERS = (4, 6, 8)
+
+
+def batch_draw(seed, bottom=None, sizes=BATCH_SIZES, nzs=BATCH_NZ, orders=BATCH_ORDERS):
+    """Scalar parameters of one tank of the many-tank workload, drawn from its own seed: Nx and Ny
+    independently from `sizes`, Nz from `nzs`, stencil order from `orders` (redrawn while the sigma
+    stencil, order + 1 points, is wider than the Nz + 1 levels: order 8 needs Nz >= 8), depth
+    h0 ~ U[0.5, 2], k_p h0 ~ U[0.5, 3], Hs / h0 ~ U[0.005, 0.03]; bottom "flat", "shoal" (Gaussian shoal of
+    half the depth) or None (drawn)."""
+    rng = np.random.default_rng(seed)
+    nx = int(rng.choice(sizes))
+    ny = int(rng.choice(sizes))
+    nz = int(rng.choice(nzs))
+    order = int(rng.choice(orders))
+    while order + 1 > nz + 1:
+        order = int(rng.choice(orders))
+    h0 = float(rng.uniform(0.5, 2.0))
+    kph = float(rng.uniform(0.5, 3.0))
+    hs = float(rng.uniform(0.005, 0.03)) * h0
+    if bottom is None:
+        bottom = "shoal" if rng.uniform() < 0.5 else "flat"
+    return dict(nx=nx, ny=ny, nz=nz, order=order, h0=h0, kph=kph, hs=hs, bottom=bottom,
+                wave_seed=int(rng.integers(1 << 31)))
+
+
+def batch_cfg(d):
+    """Tank configuration of a draw: 16 points per peak wavelength (L = (N - 1) / 16 * lambda_p)."""
+    lam = 2 * np.pi * d["h0"] / d["kph"]
+    a = d["order"] // 2
+    return dict(nx=d["nx"], ny=d["ny"], nz=d["nz"], lx=(d["nx"] - 1) / 16 * lam, ly=(d["ny"] - 1) / 16 * lam,
+                alpha=a, beta=a, gamma=a)
+
+
+def batch_tank_from(d):
+    """(cfg, bathymetry, E, P, dt, zones) of a draw: JONSWAP gamma = 3.3, 64 components, dt = 0.5 dx / c_p,
+    walls, no sponge."""
+    cfg = batch_cfg(d)
+    nx, ny, lx, ly = cfg["nx"], cfg["ny"], cfg["lx"], cfg["ly"]
+    h0 = d["h0"]
+    bath = gaussian_shoal(nx, ny, lx, ly, h0=h0) if d["bottom"] == "shoal" else flat_bottom(nx, ny, lx, ly, h0=h0)
+    E, P, info = random_waves(nx, ny, lx, ly, h0=h0, hs=d["hs"], kph=d["kph"], seed=d["wave_seed"])
+    dt = 0.5 * (lx / (nx - 1)) / info["cp"]
+    return cfg, bath, E, P, dt, []
+
+
+def batch_tank(seed, bottom=None, sizes=BATCH_SIZES, nzs=BATCH_NZ, orders=BATCH_ORDERS):
+    """One tank of the many-tank workload (batch_draw, then batch_tank_from)."""
+    return batch_tank_from(batch_draw(seed, bottom, sizes, nzs, orders))

@@ -1119,12 +1119,39 @@ static void ghost_relax(const level_t *L, double *x, const double *b) {
         }
 }
 
+/* Directions halved from an (nx, ny, nz) level: coarsening strategy 0 (MGPreProcess.f90:140-178).  Where
+ * it would halve an even column count (the reference stops with an error, MGPreProcess.f90:183-211) the
+ * column is kept and x / y are halved while they have an odd count above 3 (the GPU build's rule for
+ * even Nz).  Returns 0 when nothing can be halved. */
+static int coarsen_flags(int nx, int ny, int nz, int *xc, int *yc, int *zc) {
+    *xc = *yc = *zc = 0;
+    if (nx == 1) { if (ny == nz) { *yc = 1; *zc = 1; } else *yc = 1; }
+    else if (ny == 1) { if (nx == nz) { *xc = 1; *zc = 1; } else *xc = 1; }
+    else if (nx == ny && nx == nz) { *xc = *yc = *zc = 1; }
+    else { if (nx > nz) *xc = 1; if (ny > nz) *yc = 1; if (nx < nz && ny < nz) *zc = 1; }
+    if (*zc && nz % 2 == 0) {
+        *zc = 0;
+        *xc = (nx > 3 && nx % 2 == 1);
+        *yc = (ny > 3 && ny % 2 == 1);
+    }
+    return *xc || *yc || *zc;
+}
+
 /* MGPreProcess.f90:30-318 (coarsening strategy 0) and 321-439 (operators, ghost maps). */
 static int mg_setup(owo_tank *t) {
     const grid_t *fg = &t->g;
     int L = no_grids(fg->nx, fg->ny, fg->nz);
     if (t->p.mg_maxgrids > 0 && L > t->p.mg_maxgrids) L = t->p.mg_maxgrids;
     if (L < 1) L = 1;
+    {   /* stop where nothing can be halved */
+        int nx = fg->nx, ny = fg->ny, nz = fg->nz, xc, yc, zc;
+        for (int q = 1; q < L; q++) {
+            if (!coarsen_flags(nx, ny, nz, &xc, &yc, &zc)) { L = q; break; }
+            if (xc) nx = (nx - 1) / 2 + 1;
+            if (yc) ny = (ny - 1) / 2 + 1;
+            if (zc) nz = (nz - 1) / 2 + 1;
+        }
+    }
     t->nlev = L;
     t->lev = xcalloc(L, sizeof(level_t));
     /* finest level: fine grid with 2nd-order stencils */
@@ -1136,10 +1163,7 @@ static int mg_setup(owo_tank *t) {
         } else {
             const grid_t *f = &t->lev[k + 1].g;
             int NFx = f->nx, NFy = f->ny, NFz = f->nz, xc = 0, yc = 0, zc = 0;
-            if (NFx == 1) { if (NFy == NFz) { yc = 1; zc = 1; } else yc = 1; }
-            else if (NFy == 1) { if (NFx == NFz) { xc = 1; zc = 1; } else xc = 1; }
-            else if (NFx == NFy && NFx == NFz) { xc = yc = zc = 1; }
-            else { if (NFx > NFz) xc = 1; if (NFy > NFz) yc = 1; if (NFx < NFz && NFy < NFz) zc = 1; }
+            coarsen_flags(NFx, NFy, NFz, &xc, &yc, &zc);
             if (xc && NFx % 2 == 0) { fprintf(stderr, "ow3d_oracle: multigrid needs odd Nx (got %d)\n", NFx); return -1; }
             if (yc && NFy % 2 == 0) { fprintf(stderr, "ow3d_oracle: multigrid needs odd Ny (got %d)\n", NFy); return -1; }
             g->nx = xc ? (NFx - 1) / 2 + 1 : NFx;
