@@ -7,6 +7,7 @@ reference's count, OceanWave3D.f90:29).  The timed region covers exactly --steps
 state resident in HBM, bracketed by device synchronisation.
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -82,6 +83,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-counter runs of the matvec")
     ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
                     help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="many-tank workload instead: N uneven tanks advanced together (prints its own JSON line)")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
+                    help="request-loop workload instead: Poisson arrivals served for SECONDS (prints its own JSON line)")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
                     help="only run K high-order matvec launches of the workload and a 256 MiB device copy "
                          "(the calibration kernel for HBM counters), then exit")
@@ -95,6 +100,12 @@ def main():
     from ow3d import problems
     if args.profile_matvec:
         profile_matvec(ow3d, problems, args.profile_matvec)
+        return
+    if args.batch:
+        print(json.dumps(bench_batch(ow3d, problems, args.batch, args.steps, args.warmup)), flush=True)
+        return
+    if args.serve:
+        print(json.dumps(bench_serve(args.serve)), flush=True)
         return
     if args.profile_precond:
         T, cfg, dt = make_tank(ow3d, problems, 1)
@@ -161,6 +172,75 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0):
+    """Many-tank workload: ntanks tanks drawn as problems.batch_draw (Nx, Ny in {65..513}, Nz in {6, 9, 12},
+    order 4/6/8, own depth, spectrum and seed), advanced together (ow3d.batch_rk4_steps: each tank on its
+    own stream, steps replayed from captured graphs) and then one tank at a time, FP64 and FP32 multigrid.
+    Reports DOF-steps/s of both and the largest eta difference between the precisions."""
+    draws = [problems.batch_draw(seed * 1000003 + i) for i in range(ntanks)]
+    res = {}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        tanks, dts, dof = [], [], 0
+        t_build = time.perf_counter()
+        for d in draws:
+            cfg, bath, E, P, dt, _ = problems.batch_tank_from(d)
+            cfg = dict(cfg, mg_precision=prec, gmres_m=23, reltol=1e-8, abstol=0.0)
+            T = ow3d.Tank(cfg, **bath)
+            T.set_state(E, P)
+            tanks.append(T); dts.append(dt)
+            dof += cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
+        t_build = time.perf_counter() - t_build
+        print(f"batch {label}: {ntanks} tanks, {dof} DOF built in {t_build:.1f} s", file=sys.stderr, flush=True)
+        ow3d.batch_rk4_steps(tanks, warmup, dts)
+        a = time.perf_counter()
+        st = ow3d.batch_rk4_steps(tanks, steps, dts)
+        t_batch = time.perf_counter() - a
+        print(f"batch {label}: {steps} batched steps in {t_batch:.2f} s", file=sys.stderr, flush=True)
+        t_seq = 0.0
+        for T, dt in zip(tanks, dts):
+            a = time.perf_counter()
+            T.rk4_steps(steps, 0.0, dt)
+            t_seq += time.perf_counter() - a
+        E_end = [T.get_state()[0] for T in tanks]
+        conv = [T.rk4_steps(1, 0.0, dt) == 0 for T, dt in zip(tanks, dts)]   # one more step: did every solve converge
+        res[label] = dict(dof_steps_per_s=dof * steps / t_batch, one_at_a_time_dof_steps_per_s=dof * steps / t_seq,
+                          batch_gain=t_seq / t_batch, ms_per_batch_step=1e3 * t_batch / steps, status=st,
+                          host_setup_s=t_build, E=E_end, conv=conv,
+                          replays=sum(T.step_stats()[0] for T in tanks), captures=sum(T.step_stats()[1] for T in tanks))
+        for T in tanks:
+            T.close()
+    errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(res["mixed"]["E"], res["fp64"]["E"])]
+    conv = [a and b for a, b in zip(res["mixed"]["conv"], res["fp64"]["conv"])]
+    err_conv = max([e for e, c in zip(errs, conv) if c], default=None)
+    for r in res.values():
+        del r["E"], r["conv"]
+    nz = collections.Counter((d["nz"], d["order"]) for d in draws)
+    return {"workload": f"{ntanks} uneven tanks (Nx, Ny in 65..513, Nz 6/9/12, order 4/6/8), seed {seed}",
+            "dof": sum(d["nx"] * d["ny"] * (d["nz"] + 1) for d in draws), "steps": steps, "warmup": warmup,
+            "nz_order_mix": {f"{k[0]}/{k[1]}": v for k, v in sorted(nz.items())},
+            "fp64": res["fp64"], "mixed": res["mixed"],
+            "speedup_mixed_batched_vs_fp64_batched": res["mixed"]["dof_steps_per_s"] / res["fp64"]["dof_steps_per_s"],
+            "speedup_mixed_batched_vs_fp64_one_at_a_time": res["mixed"]["dof_steps_per_s"] / res["fp64"]["one_at_a_time_dof_steps_per_s"],
+            "rel_err_eta_mixed_vs_fp64_max_over_tanks": max(errs),
+            "rel_err_eta_mixed_vs_fp64_max_over_converged_tanks": err_conv,
+            "tanks_with_unconverged_solves": int(len(conv) - sum(conv))}
+
+
+def bench_serve(seconds, rate=20.0, max_active=32, seed=0):
+    """Request-loop workload: Poisson arrivals (rate per second) of many-tank draws with 50..500 steps and
+    a flat or shoal bottom, admitted by a 200 GB HBM budget and at most max_active tanks at once, served
+    for `seconds` with the FP64 and then the FP32 multigrid."""
+    from ow3d import serve
+    out = {"workload": f"Poisson {rate}/s tank requests, 50..500 steps, {max_active} slots, 200 GB budget, {seconds} s"}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        reqs = serve.poisson_requests(int(rate * seconds * 2) + 8, rate, seed, mg_precision=prec)
+        out[label] = serve.Server(hbm_budget=200e9, max_active=max_active).run(reqs, time_limit=seconds)
+        print(f"serve {label}: {out[label]['steps']} steps, {out[label]['completed']} requests completed",
+              file=sys.stderr, flush=True)
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["dof_steps_per_s"] / out["fp64"]["dof_steps_per_s"]
+    return out
 
 
 def profile_matvec(ow3d, problems, k):

@@ -11,7 +11,9 @@
  * (the reference ignores this, iterative_solution.f90:115-134; here it is reported),
  * <0 error (ow3d_last_error() gives the message).
  *
- * One handle per host thread; each handle owns its device buffers and its HIP stream.
+ * One handle per host thread at a time; each handle owns its device buffers and its HIP stream.
+ * Handles may be created, destroyed and have state copied on one thread while others step theirs:
+ * graph capture, creation, destruction and state copies are serialised by a process-wide lock.
  */
 #ifndef OW3D_H
 #define OW3D_H

@@ -14,6 +14,11 @@ std::string &ow3d_host::err() {
 
 extern "C" const char *ow3d_last_error(void) { return ow3d_host::err().c_str(); }
 
+std::mutex &ow3d_host::capture_lock() {
+    static std::mutex m;
+    return m;
+}
+
 std::atomic<long long> &ow3d_host::device_bytes() {
     static std::atomic<long long> b{0};
     return b;
@@ -183,6 +188,7 @@ __global__ void k_extrap3(size_t n, const double *__restrict__ Lnew, const doubl
 extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double *hx, const double *hy, const double *hxx,
                            const double *hyy, ow3d_tank **out) {
     if (!cfg || !h || !out) { err() = "null argument"; return -1; }
+    std::lock_guard<std::mutex> lk(capture_lock());
     *out = nullptr;
     const ow3d_config &c = *cfg;
     if (c.nx < 2 || c.ny < 1 || c.nz < 3) { err() = "need Nx >= 2, Ny >= 1, Nz >= 3"; return -1; }
@@ -287,6 +293,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
 
 extern "C" void ow3d_destroy(ow3d_tank *t) {
     if (!t) return;
+    std::lock_guard<std::mutex> lk(capture_lock());
     (void)hipStreamSynchronize(t->s);
     delete t;
 }
@@ -428,6 +435,7 @@ extern "C" int ow3d_add_relax_zone(ow3d_tank *t, int axis, int i0, int i1, int j
     if (i0 < 0 || j0 < 0 || i1 >= t->g.nxg || j1 >= t->g.nyg || i1 < i0 || j1 < j0 || (axis != 0 && axis != 1)) {
         err() = "relaxation zone outside the grid"; return -1;
     }
+    std::lock_guard<std::mutex> lk(capture_lock());
     HIPCHK(hipStreamSynchronize(t->s));
     t->drop_step_graphs();   // captured steps hold the old zone arrays
     Zone z{axis, i0, i1, j0, j1, (int)t->gam.size()};
@@ -443,6 +451,7 @@ extern "C" int ow3d_add_relax_zone(ow3d_tank *t, int axis, int i0, int i1, int j
 
 extern "C" int ow3d_set_state(ow3d_tank *t, const double *E, const double *P, const double *W, const double *phi) {
     if (!t || !E || !P) { err() = "null argument"; return -1; }
+    std::lock_guard<std::mutex> lk(capture_lock());
     const size_t nxy = t->g.nxy, n = t->g.n;
     hipStream_t s = t->s;
     HIPCHK(hipMemcpyAsync(t->E.p, E, nxy * 8, hipMemcpyHostToDevice, s));
@@ -462,6 +471,7 @@ extern "C" int ow3d_set_state(ow3d_tank *t, const double *E, const double *P, co
 
 extern "C" int ow3d_get_state(ow3d_tank *t, double *E, double *P, double *W, double *phi) {
     if (!t) return -1;
+    std::lock_guard<std::mutex> lk(capture_lock());
     const size_t nxy = t->g.nxy, n = t->g.n;
     hipStream_t s = t->s;
     if (E) HIPCHK(hipMemcpyAsync(E, t->E.p, nxy * 8, hipMemcpyDeviceToHost, s));
@@ -526,6 +536,7 @@ static int step_launch(ow3d_tank *t, double dt, const int *slots) {
         auto it = t->graphs.find(key);
         if (it == t->graphs.end()) {
             const auto c0 = std::chrono::steady_clock::now();
+            std::lock_guard<std::mutex> lk(capture_lock());
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(t->s, hipStreamCaptureModeThreadLocal));
             t->capturing = true;

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,11 @@ constexpr int RED_BLOCKS = 2048;
 
 // bytes of device memory held by the library's buffers (all handles of the process)
 std::atomic<long long> &device_bytes();
+
+// Process-wide lock held while a stream is captured into a graph and while a handle is created,
+// destroyed or has state copied in or out (allocation, frees and blocking copies on other threads must
+// not run during a capture).
+std::mutex &capture_lock();
 
 template <typename T>
 struct DevBuf {

@@ -44,6 +44,7 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
     }
     if (t->use_graph && !t->capturing && r == t->w.p && s == t->s) {
         if (!t->pg_exec) {
+            std::lock_guard<std::mutex> lk(capture_lock());
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s));

@@ -165,3 +165,26 @@ def test_speculative_steps_replay_to_the_whole_budget_result():
     assert Tf.stats() == Ts.stats()
     assert Tf.replays() == 0
     assert Ts.replays() >= 6, Ts.replays()
+
+
+def test_request_loop_results_equal_each_request_alone():
+    """Requests arriving while others run are admitted as slots free up (2 slots for 6 requests), their
+    steps overlap on the device, and every request's final state equals its run alone, bit for bit."""
+    from ow3d import serve
+    reqs = serve.poisson_requests(6, rate=200.0, seed=5, mg_precision=1, steps=(3, 9), sizes=(17, 33, 65))
+    srv = serve.Server(hbm_budget=50e9, max_active=2, keep_results=True)
+    out = srv.run(reqs)
+    assert out["completed"] == 6 and out["admitted"] == 6
+    assert out["steps"] == sum(r.nsteps for r in reqs)
+    assert out["queue_delay_s"]["p99"] > 0.0          # some requests waited for a slot
+    for r in reqs:
+        assert same(out["results"][r.rid], serve.run_alone(r)), r.rid
+
+
+def test_request_loop_respects_the_memory_budget():
+    from ow3d import serve
+    reqs = serve.poisson_requests(4, rate=1000.0, seed=6, mg_precision=1, steps=(2, 3), sizes=(33,))
+    budget = max(r.bytes for r in reqs) * 1.5         # room for one tank at a time
+    out = serve.Server(hbm_budget=budget, max_active=8).run(reqs)
+    assert out["completed"] == 4
+    assert out["queue_delay_s"]["p99"] > 0.0
