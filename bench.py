@@ -22,12 +22,15 @@ WORKLOAD = "shoal513"
 HBM_PEAK_GBS = 8000.0
 
 
+MG_LEVELS = 7
+
+
 def make_tank(ow3d, problems, mg_precision, solver=1):
     cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD)
     cfg = dict(cfg)
-    # 8 multigrid levels (MGmaxgrids): 513^2 .. 9^2 by semi-coarsening in x and y, then the 5^3 level,
-    # which is solved exactly
-    cfg.update(mg_precision=mg_precision, solver=solver, reltol=1e-8, abstol=0.0, gmres_m=23, mg_maxgrids=8)
+    # MGmaxgrids: 8 levels are 513^2 .. 9^2 by semi-coarsening in x and y, then the 5^3 level, which is
+    # solved exactly; 7 levels stop at 17^2 x 9 (3,610 unknowns, solved exactly)
+    cfg.update(mg_precision=mg_precision, solver=solver, reltol=1e-8, abstol=0.0, gmres_m=23, mg_maxgrids=MG_LEVELS)
     T = ow3d.Tank(cfg, **bath)
     for z in zones:
         T.add_relax_zone(*z)
@@ -75,6 +78,7 @@ def cpu_baseline(cfg_gpu, budget_s=25.0):
 
 
 def main():
+    global MG_LEVELS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -83,6 +87,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-counter runs of the matvec")
     ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
                     help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
+    ap.add_argument("--mg-levels", type=int, default=MG_LEVELS, help="multigrid levels of the 513^2 tank (MGmaxgrids)")
+    ap.add_argument("--batched-tanks", type=int, default=2,
+                    help="also advance this many 513x513x9 tanks together (different JONSWAP seeds); 0: skip")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="many-tank workload instead: N uneven tanks advanced together (prints its own JSON line)")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
@@ -98,6 +105,7 @@ def main():
     tr = None if (args.no_traffic or args.profile_matvec or args.profile_precond) else measure_traffic()
     import ow3d
     from ow3d import problems
+    MG_LEVELS = args.mg_levels
     if args.profile_matvec:
         profile_matvec(ow3d, problems, args.profile_matvec)
         return
@@ -135,6 +143,7 @@ def main():
         T.close()
     dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
     mix, f64 = results["mixed"], results["fp64"]
+    batched = bench_batched_513(ow3d, problems, args.batched_tanks, args.steps, args.warmup) if args.batched_tanks > 1 else None
     t_mix = sum(mix["lat"])
     t_f64 = sum(f64["lat"])
     err_eta = float(np.max(np.abs(mix["E"] - f64["E"])) / np.max(np.abs(f64["E"])))
@@ -154,7 +163,7 @@ def main():
         "dtype": "f64 (FP32 multigrid preconditioner)",
         "data": "synthetic",
         "config": {"workload": "513x513x9 Gaussian shoal, 6th order, 5-wavelength sponges, JONSWAP seed 0",
-                   "dof": dof, "solver": "GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, 8 levels"},
+                   "dof": dof, "solver": f"GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, {MG_LEVELS} levels"},
         "fp64": {"value": dof * args.steps / t_f64, "ms_per_step": 1e3 * t_f64 / args.steps,
                  "p50_ms": 1e3 * float(np.percentile(f64["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(f64["lat"], 99)),
                  "iters": f64["iters"], "step_graphs": f64["graphs"]},
@@ -166,12 +175,43 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": mv_ms * 1e3, "algorithmic_bytes": b_mv},
     }
+    if batched is not None:
+        batched["mixed_batched_vs_plain_fp64"] = batched["mixed"]["dof_steps_per_s"] / out["fp64"]["value"]
+        out["batched"] = batched
     if tr is not None:
         out["roofline"]["traffic"] = tr["bytes"]
         out["roofline"]["traffic_detail"] = tr
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
+
+
+def bench_batched_513(ow3d, problems, k, steps, warmup):
+    """k tanks of the 513x513x9 workload (JONSWAP seeds 0 .. k-1) advanced together with
+    ow3d.batch_rk4_steps, FP64 and FP32 multigrid: DOF-steps/s of the whole batch and its step latency."""
+    out = {"tanks": k}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        tanks, dts = [], []
+        for s in range(k):
+            cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD, seed=s)
+            cfg = dict(cfg, mg_precision=prec, solver=1, reltol=1e-8, abstol=0.0, gmres_m=23, mg_maxgrids=MG_LEVELS)
+            T = ow3d.Tank(cfg, **bath)
+            for z in zones:
+                T.add_relax_zone(*z)
+            T.set_state(E, P)
+            tanks.append(T); dts.append(dt)
+        ow3d.batch_rk4_steps(tanks, warmup, dts)
+        lat = []
+        for _ in range(steps):
+            a = time.perf_counter()
+            ow3d.batch_rk4_steps(tanks, 1, dts)
+            lat.append(time.perf_counter() - a)
+        dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1) * k
+        out[label] = {"dof_steps_per_s": dof * steps / sum(lat), "ms_per_batch_step": 1e3 * sum(lat) / steps,
+                      "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99))}
+        for T in tanks:
+            T.close()
+    return out
 
 
 def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0):

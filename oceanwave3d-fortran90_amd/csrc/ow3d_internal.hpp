@@ -114,6 +114,8 @@ struct ow3d_tank {
     bool fused_rb = true;             // one launch per zebra sweep (red and black halves) on the upper levels
     BottomArgs<double> bot64{};
     BottomArgs<float> bot32{};
+    DevBuf<BottomArgs<double>> bot64d;   // device copies read by the bottom kernel
+    DevBuf<BottomArgs<float>> bot32d;
     // device-driven solves: scalar state, statistics and the done flag every solve kernel reads
     DevBuf<KrylovState> ks;
     const int *done() const { return &ks.p->done; }
@@ -168,8 +170,8 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s);
 // per-precision parts (ow3d_mg64.hip, ow3d_mg32.hip)
 int mg_build64(ow3d_tank *t);
 int mg_build32(ow3d_tank *t);
-void mg_fill_bottom64(ow3d_tank *t);
-void mg_fill_bottom32(ow3d_tank *t);
+int mg_fill_bottom64(ow3d_tank *t);
+int mg_fill_bottom32(ow3d_tank *t);
 int precond_body64(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
 int precond_body32(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
 }  // namespace ow3d_host

@@ -2030,32 +2030,47 @@ __device__ void bottom_smooth(const LvArgs<T> &L, int nu, T *x, const T *b) {
     }
 }
 
-// All levels 1 .. nlev-1 of the bottom group have NZ sigma levels; level 0 is solved densely.
+// The V cycle on the small levels in one workgroup (a workgroup barrier between steps instead of a
+// kernel boundary).  All levels 1 .. nlev-1 of the group have NZ sigma levels; level 0 is solved
+// densely.  The level table lives in device memory: each level's arguments are copied out of it once
+// (no run-time indexing of a kernel-argument array, which would go through scratch).
 template <typename T, int NZ>
-__global__ __launch_bounds__(512) void k_mg_bottom(BottomArgs<T> A) {
-    if (solve_done(A.skip)) return;
-    const int top = A.nlev - 1;
+__global__ __launch_bounds__(512) void k_mg_bottom(const BottomArgs<T> *__restrict__ A) {
+    if (solve_done(A->skip)) return;
+    const int top = A->nlev - 1, nu1 = A->nu1, nu2 = A->nu2;
     for (int l = top; l >= 1; l--) {
-        bottom_smooth<T, NZ>(A.L[l], A.nu1, A.x[l], A.b[l]);
-        bottom_residual<T, NZ>(A.L[l], A.x[l], A.b[l], A.r[l]);
+        const LvArgs<T> L = A->L[l];
+        T *x = A->x[l], *r = A->r[l];
+        const T *b = A->b[l];
+        bottom_smooth<T, NZ>(L, nu1, x, b);
+        bottom_residual<T, NZ>(L, x, b, r);
         __syncthreads();
-        const RestrictGeo &G = A.G[l];
+        const RestrictGeo G = A->G[l];
+        T *bc = A->b[l - 1], *xc = A->x[l - 1];
         const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
-        for (size_t q = threadIdx.x; q < nc; q += blockDim.x) restrict_point(G, q, A.r[l], A.b[l - 1], A.x[l - 1]);
+        for (size_t q = threadIdx.x; q < nc; q += blockDim.x) restrict_point(G, q, (const T *)r, bc, xc);
         __syncthreads();
     }
-    for (int rr = threadIdx.x; rr < A.n0; rr += blockDim.x) {
-        T s = T(0);
-        for (int c = 0; c < A.n0; c++) s += A.minv[(size_t)rr * A.n0 + c] * A.b[0][c];
-        A.x[0][rr] = s;
+    {
+        const int n0 = A->n0;
+        const T *minv = A->minv, *b0 = A->b[0];
+        T *x0 = A->x[0];
+        for (int rr = threadIdx.x; rr < n0; rr += blockDim.x) {
+            T s = T(0);
+            for (int c = 0; c < n0; c++) s += minv[(size_t)rr * n0 + c] * b0[c];
+            x0[rr] = s;
+        }
     }
     __syncthreads();
     for (int l = 1; l <= top; l++) {
-        const RestrictGeo &G = A.G[l];
+        const RestrictGeo G = A->G[l];
+        const LvArgs<T> L = A->L[l];
+        T *x = A->x[l];
+        const T *b = A->b[l], *xc = A->x[l - 1];
         const size_t nf = (size_t)G.nxgf * G.nygf * G.nzgf;
-        for (size_t q = threadIdx.x; q < nf; q += blockDim.x) prolong_point(G, q, A.x[l - 1], A.x[l]);
+        for (size_t q = threadIdx.x; q < nf; q += blockDim.x) prolong_point(G, q, xc, x);
         __syncthreads();
-        bottom_smooth<T, NZ>(A.L[l], A.nu2, A.x[l], A.b[l]);
+        bottom_smooth<T, NZ>(L, nu2, x, b);
     }
 }
 

@@ -28,9 +28,7 @@ int mg_setup(ow3d_tank *t) {
         for (int q = 1; q < k; q++) same &= t->levels[q].nzg == t->levels[k].nzg;
         if (same) t->bot_top = k;
     }
-    if (c.mg_precision == 0) mg_fill_bottom64(t);
-    else mg_fill_bottom32(t);
-    return 0;
+    return c.mg_precision == 0 ? mg_fill_bottom64(t) : mg_fill_bottom32(t);
 }
 
 // u = M^-1 r (device vectors, FP64 in / out), skipped once the solve's done flag is set.  When r is the

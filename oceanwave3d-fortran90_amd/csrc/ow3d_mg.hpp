@@ -1,6 +1,7 @@
 // Multigrid preconditioner internals shared by the per-precision translation units ow3d_mg64.hip and
 // ow3d_mg32.hip (level set-up, kernel dispatch over the column depth, the V cycle).
 #pragma once
+#include <thread>
 #include "ow3d_internal.hpp"
 
 namespace ow3d_host {
@@ -54,29 +55,57 @@ int mg_launch(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, 
 // ---------------------------------------------------------------------------------------------
 // Multigrid set-up and V cycle (MultiGridSolver.f90:5-192 structure with the zebra smoother)
 // ---------------------------------------------------------------------------------------------
+// Inverse of the coarsest level operator by LU with partial pivoting.  The operator is banded (kl rows
+// below, ku above the diagonal in the sigma-fastest ordering), so elimination and substitution only
+// visit the band (pivoting widens U to ku + kl); every skipped term is an exact zero, so the result is
+// the one dense elimination gives.  The n columns of the inverse are solved on up to 16 host threads.
 inline void dense_inverse(const std::vector<double> &A, int n, std::vector<double> &Ainv) {
     std::vector<double> M(A);
+    int kl = 0, ku = 0;
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++)
+            if (M[(size_t)r * n + c] != 0.0) { kl = std::max(kl, r - c); ku = std::max(ku, c - r); }
+    const int kw = ku + kl;   // U's band after row interchanges
     std::vector<int> piv(n);
     for (int c = 0; c < n; c++) {
+        const int rend = std::min(n, c + kl + 1), qend = std::min(n, c + kw + 1);
         int pr = c; double best = std::fabs(M[(size_t)c * n + c]);
-        for (int r = c + 1; r < n; r++) if (std::fabs(M[(size_t)r * n + c]) > best) { best = std::fabs(M[(size_t)r * n + c]); pr = r; }
+        for (int r = c + 1; r < rend; r++) if (std::fabs(M[(size_t)r * n + c]) > best) { best = std::fabs(M[(size_t)r * n + c]); pr = r; }
         piv[c] = pr;
         if (pr != c) for (int q = 0; q < n; q++) std::swap(M[(size_t)c * n + q], M[(size_t)pr * n + q]);
-        for (int r = c + 1; r < n; r++) {
+        for (int r = c + 1; r < rend; r++) {
             double f = M[(size_t)r * n + c] / M[(size_t)c * n + c];
             M[(size_t)r * n + c] = f;
-            for (int q = c + 1; q < n; q++) M[(size_t)r * n + q] -= f * M[(size_t)c * n + q];
+            for (int q = c + 1; q < qend; q++) M[(size_t)r * n + q] -= f * M[(size_t)c * n + q];
         }
     }
     Ainv.assign((size_t)n * n, 0.0);
-    std::vector<double> b(n);
-    for (int col = 0; col < n; col++) {
-        std::fill(b.begin(), b.end(), 0.0); b[col] = 1.0;
-        for (int c = 0; c < n; c++) if (piv[c] != c) std::swap(b[c], b[piv[c]]);
-        for (int r = 0; r < n; r++) { double s = b[r]; for (int q = 0; q < r; q++) s -= M[(size_t)r * n + q] * b[q]; b[r] = s; }
-        for (int r = n - 1; r >= 0; r--) { double s = b[r]; for (int q = r + 1; q < n; q++) s -= M[(size_t)r * n + q] * b[q]; b[r] = s / M[(size_t)r * n + r]; }
-        for (int r = 0; r < n; r++) Ainv[(size_t)r * n + col] = b[r];
-    }
+    auto solve_cols = [&](int c0, int c1) {
+        std::vector<double> b(n);
+        for (int col = c0; col < c1; col++) {
+            std::fill(b.begin(), b.end(), 0.0); b[col] = 1.0;
+            for (int c = 0; c < n; c++) if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+            int first = 0;
+            while (first < n && b[first] == 0.0) first++;
+            for (int r = first; r < n; r++) {
+                double s = b[r];
+                for (int q = std::max(first, r - kl); q < r; q++) s -= M[(size_t)r * n + q] * b[q];
+                b[r] = s;
+            }
+            for (int r = n - 1; r >= 0; r--) {
+                double s = b[r];
+                const int qend = std::min(n, r + kw + 1);
+                for (int q = r + 1; q < qend; q++) s -= M[(size_t)r * n + q] * b[q];
+                b[r] = s / M[(size_t)r * n + r];
+            }
+            for (int r = 0; r < n; r++) Ainv[(size_t)r * n + col] = b[r];
+        }
+    };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 512 || nt == 1) { solve_cols(0, n); return; }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; k++) th.emplace_back(solve_cols, (int)((long)n * k / nt), (int)((long)n * (k + 1) / nt));
+    for (auto &x : th) x.join();
 }
 
 // Host evaluation of the level operator applied to a unit vector (coarsest level assembly).
@@ -215,9 +244,9 @@ inline RestrictGeo geo(const Grid &f, const Grid &c, const double *xf, const dou
 }
 
 template <typename T>
-void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, BottomArgs<T> &A) {
+int fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, BottomArgs<T> &A, DevBuf<BottomArgs<T>> &dA) {
     A = BottomArgs<T>{};
-    if (t->bot_top < 0) return;
+    if (t->bot_top < 0) return 0;
     A.nlev = t->bot_top + 1;
     A.nu1 = t->cfg.nu1; A.nu2 = t->cfg.nu2;
     A.n0 = (int)mg[0]->g.n;
@@ -228,6 +257,9 @@ void fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, Bot
         if (k >= 1) A.G[k] = geo(L.g, mg[k - 1]->g, L.xf.p, L.yf.p, L.zf.p, t->done());
     }
     A.skip = t->done();
+    HIPCHK(dA.alloc(1));
+    HIPCHK(hipMemcpy(dA.p, &A, sizeof(A), hipMemcpyHostToDevice));
+    return 0;
 }
 
 template <typename T>
@@ -286,9 +318,9 @@ int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s) {
     return 0;
 }
 
-template <typename T> const BottomArgs<T> &bottom_args(const ow3d_tank *t);
-template <> inline const BottomArgs<double> &bottom_args<double>(const ow3d_tank *t) { return t->bot64; }
-template <> inline const BottomArgs<float> &bottom_args<float>(const ow3d_tank *t) { return t->bot32; }
+template <typename T> const BottomArgs<T> *bottom_args(const ow3d_tank *t);
+template <> inline const BottomArgs<double> *bottom_args<double>(const ow3d_tank *t) { return t->bot64d.p; }
+template <> inline const BottomArgs<float> *bottom_args<float>(const ow3d_tank *t) { return t->bot32d.p; }
 
 // V cycle on level k from its current iterate `cur` (on entry L.x for every level below the finest);
 // on return `cur` points at the buffer holding the result.

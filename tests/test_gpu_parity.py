@@ -93,6 +93,15 @@ def test_precond_fp64_parity(oracle_mod, nx, ny, nz, shoal):
     assert rel(ug, uo) < 1e-10
 
 
+def test_precond_with_a_large_coarsest_level(oracle_mod):
+    """Two levels: the coarsest is 17 x 17 x 9 (3,610 unknowns), inverted on the host by banded LU on
+    several threads; the V cycle must match the oracle's (dense LU coarse solve)."""
+    T_o, T_g, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_maxgrids=2)
+    assert T_g.mg_levels == 2
+    r = np.random.default_rng(8).standard_normal(T_o.shape3)
+    assert rel(T_g.precond_apply(r), T_o.precond(r)) < 1e-10
+
+
 def test_precond_fp32(oracle_mod):
     T_o, T_g, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=1)
     r = np.random.default_rng(6).standard_normal(T_o.shape3)
