@@ -22,16 +22,26 @@ WORKLOAD = "shoal513"
 HBM_PEAK_GBS = 8000.0
 
 
-MG_LEVELS = 7
+MG_LEVELS = None   # None: the autotuner's cached schedule for the class, else 7 levels
 
 
-def make_tank(ow3d, problems, mg_precision, solver=1):
-    cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD)
-    cfg = dict(cfg)
-    # MGmaxgrids: 8 levels are 513^2 .. 9^2 by semi-coarsening in x and y, then the 5^3 level, which is
-    # solved exactly; 7 levels stop at 17^2 x 9 (3,610 unknowns, solved exactly)
-    cfg.update(mg_precision=mg_precision, solver=solver, reltol=1e-8, abstol=0.0, gmres_m=23, mg_maxgrids=MG_LEVELS)
+def tank_cfg(cfg, mg_precision, solver=1):
+    """The 513^2 tank's solver settings and, unless --mg-levels is given, the autotuner's cached schedule
+    (ow3d/tuning.json) for its class; without a cached entry 7 levels (513^2 .. 9^2 x 9, whose 1,210
+    unknowns are solved exactly) and the default launch choices."""
+    from ow3d import autotune
+    cfg = dict(cfg, mg_precision=mg_precision, solver=solver, reltol=1e-8, abstol=0.0, gmres_m=23)
+    choice = autotune.tuned(cfg) if MG_LEVELS is None else None
+    cfg["mg_maxgrids"] = choice["mg_maxgrids"] if choice else (MG_LEVELS or 7)
+    return cfg, choice
+
+
+def make_tank(ow3d, problems, mg_precision, solver=1, seed=0):
+    from ow3d import autotune
+    cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD, seed=seed)
+    cfg, choice = tank_cfg(cfg, mg_precision, solver)
     T = ow3d.Tank(cfg, **bath)
+    autotune.apply(T, choice)
     for z in zones:
         T.add_relax_zone(*z)
     T.set_state(E, P)
@@ -87,11 +97,13 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-counter runs of the matvec")
     ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
                     help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
-    ap.add_argument("--mg-levels", type=int, default=MG_LEVELS, help="multigrid levels of the 513^2 tank (MGmaxgrids)")
+    ap.add_argument("--mg-levels", type=int, default=None,
+                    help="multigrid levels of the 513^2 tank (MGmaxgrids); default: the autotuner's cached choice, else 7")
     ap.add_argument("--batched-tanks", type=int, default=2,
                     help="also advance this many 513x513x9 tanks together (different JONSWAP seeds); 0: skip")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="many-tank workload instead: N uneven tanks advanced together (prints its own JSON line)")
+    ap.add_argument("--untuned", action="store_true", help="--batch without the autotuner's cached schedules")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
                     help="request-loop workload instead: Poisson arrivals served for SECONDS (prints its own JSON line)")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
@@ -110,7 +122,8 @@ def main():
         profile_matvec(ow3d, problems, args.profile_matvec)
         return
     if args.batch:
-        print(json.dumps(bench_batch(ow3d, problems, args.batch, args.steps, args.warmup)), flush=True)
+        print(json.dumps(bench_batch(ow3d, problems, args.batch, args.steps, args.warmup, use_tuning=not args.untuned)),
+              flush=True)
         return
     if args.serve:
         print(json.dumps(bench_serve(args.serve)), flush=True)
@@ -140,6 +153,9 @@ def main():
             mv_ms = T.time_matvec(200)
             ncol = T.nxg * T.nyg
             b_mv = 16 * T.n + 8 * 5 * ncol
+            from ow3d import autotune
+            ch = autotune.tuned(cfg) if MG_LEVELS is None else None
+            mv_kernel = "k_matvec3p" if ch and ch["matvec_variant"] == 2 else "k_matvec3c"
         T.close()
     dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
     mix, f64 = results["mixed"], results["fp64"]
@@ -163,7 +179,7 @@ def main():
         "dtype": "f64 (FP32 multigrid preconditioner)",
         "data": "synthetic",
         "config": {"workload": "513x513x9 Gaussian shoal, 6th order, 5-wavelength sponges, JONSWAP seed 0",
-                   "dof": dof, "solver": f"GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, {MG_LEVELS} levels"},
+                   "dof": dof, "solver": f"GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, {cfg['mg_maxgrids']} levels (mixed run)"},
         "fp64": {"value": dof * args.steps / t_f64, "ms_per_step": 1e3 * t_f64 / args.steps,
                  "p50_ms": 1e3 * float(np.percentile(f64["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(f64["lat"], 99)),
                  "iters": f64["iters"], "step_graphs": f64["graphs"]},
@@ -171,7 +187,7 @@ def main():
                   "iters": mix["iters"], "step_graphs": mix["graphs"], "rel_err_eta_vs_fp64": err_eta,
                   "rel_err_phi_vs_fp64": err_phi},
         "speedup_mixed_vs_fp64": t_f64 / t_mix,
-        "roofline": {"bound": "hbm", "kernel": "k_matvec3c (high-order sigma-Laplace matvec)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"{mv_kernel} (high-order sigma-Laplace matvec)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": mv_ms * 1e3, "algorithmic_bytes": b_mv},
     }
@@ -193,12 +209,7 @@ def bench_batched_513(ow3d, problems, k, steps, warmup):
     for label, prec in (("fp64", 0), ("mixed", 1)):
         tanks, dts = [], []
         for s in range(k):
-            cfg, bath, E, P, dt, zones = problems.workload(WORKLOAD, seed=s)
-            cfg = dict(cfg, mg_precision=prec, solver=1, reltol=1e-8, abstol=0.0, gmres_m=23, mg_maxgrids=MG_LEVELS)
-            T = ow3d.Tank(cfg, **bath)
-            for z in zones:
-                T.add_relax_zone(*z)
-            T.set_state(E, P)
+            T, cfg, dt = make_tank(ow3d, problems, prec, seed=s)
             tanks.append(T); dts.append(dt)
         ow3d.batch_rk4_steps(tanks, warmup, dts)
         lat = []
@@ -214,20 +225,28 @@ def bench_batched_513(ow3d, problems, k, steps, warmup):
     return out
 
 
-def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0):
+def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0, use_tuning=True):
     """Many-tank workload: ntanks tanks drawn as problems.batch_draw (Nx, Ny in {65..513}, Nz in {6, 9, 12},
     order 4/6/8, own depth, spectrum and seed), advanced together (ow3d.batch_rk4_steps: each tank on its
     own stream, steps replayed from captured graphs) and then one tank at a time, FP64 and FP32 multigrid.
     Reports DOF-steps/s of both and the largest eta difference between the precisions."""
+    from ow3d import autotune
+    table = autotune.load() if use_tuning else {}
     draws = [problems.batch_draw(seed * 1000003 + i) for i in range(ntanks)]
     res = {}
+    n_tuned = 0
     for label, prec in (("fp64", 0), ("mixed", 1)):
         tanks, dts, dof = [], [], 0
         t_build = time.perf_counter()
         for d in draws:
             cfg, bath, E, P, dt, _ = problems.batch_tank_from(d)
             cfg = dict(cfg, mg_precision=prec, gmres_m=23, reltol=1e-8, abstol=0.0)
+            choice = autotune.tuned(cfg, table)
+            if choice:
+                cfg["mg_maxgrids"] = choice["mg_maxgrids"]
+                n_tuned += prec
             T = ow3d.Tank(cfg, **bath)
+            autotune.apply(T, choice)
             T.set_state(E, P)
             tanks.append(T); dts.append(dt)
             dof += cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
@@ -258,6 +277,7 @@ def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0):
         del r["E"], r["conv"]
     nz = collections.Counter((d["nz"], d["order"]) for d in draws)
     return {"workload": f"{ntanks} uneven tanks (Nx, Ny in 65..513, Nz 6/9/12, order 4/6/8), seed {seed}",
+            "tanks_with_tuned_schedules": n_tuned,
             "dof": sum(d["nx"] * d["ny"] * (d["nz"] + 1) for d in draws), "steps": steps, "warmup": warmup,
             "nz_order_mix": {f"{k[0]}/{k[1]}": v for k, v in sorted(nz.items())},
             "fp64": res["fp64"], "mixed": res["mixed"],

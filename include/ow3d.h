@@ -121,9 +121,17 @@ int ow3d_wait(ow3d_tank *t);
  * so results equal those of whole-budget steps.  Stages are captured per (stage, slots).
  * Replays, graph captures and the host time spent capturing, so far: */
 int ow3d_step_stats(const ow3d_tank *t, long long *replays, long long *captures, double *capture_ms);
+/* Launch choices of a handle, set by the autotuner (ow3d/autotune.py; no counterpart in the reference):
+ * matvec_variant 0 column kernel with the widest tile fitting 64 KiB of LDS, 1 column kernel with 16 x 8
+ * tiles, 2 point kernel; sweep_threads 0 automatic (1024 below 256 tiles, else 256), 256 or 1024.
+ * Results do not depend on them. */
+int ow3d_tune(ow3d_tank *t, int matvec_variant, int sweep_threads);
 /* Device memory held by a handle, and what ow3d_create would allocate for cfg (admission control). */
 long long ow3d_device_bytes(const ow3d_tank *t);
 long long ow3d_estimate_bytes(const ow3d_config *cfg);
+/* Multigrid hierarchy ow3d_create would build for cfg: number of levels and unknowns of the coarsest
+ * (solved with its dense inverse; at most 4096).  Host only. */
+int ow3d_mg_plan(const ow3d_config *cfg, int *nlev, int *coarsest);
 
 #ifdef __cplusplus
 }

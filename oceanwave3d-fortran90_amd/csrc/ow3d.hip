@@ -206,6 +206,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
         t->speculate = !(sp && sp[0] == '0');
         // slots = most recently used + margin (a negative margin forces replays: used by the tests)
         if (const char *mg = std::getenv("OW3D_SLOT_MARGIN")) t->margin = std::atoi(mg);
+        if (const char *mv = std::getenv("OW3D_MV")) t->mv_variant = mv[0] == 'p' ? 2 : 0;
     }
     Grid &g = t->g;
     g.nx = c.nx; g.ny = c.ny; g.nz = c.nz; g.gx = 1; g.gy = c.ny > 1 ? 1 : 0;
@@ -724,6 +725,34 @@ extern "C" int ow3d_stats(const ow3d_tank *t, long long *total, long long *solve
 }
 
 extern "C" long long ow3d_device_bytes(const ow3d_tank *t) { return t ? t->bytes : -1; }
+
+extern "C" int ow3d_tune(ow3d_tank *t, int matvec_variant, int sweep_threads) {
+    if (!t) { err() = "null handle"; return -1; }
+    if (matvec_variant < 0 || matvec_variant > 2) { err() = "matvec variant must be 0, 1 or 2"; return -1; }
+    if (sweep_threads != 0 && sweep_threads != 256 && sweep_threads != 1024) { err() = "sweep threads must be 0, 256 or 1024"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    HIPCHK(hipStreamSynchronize(t->s));
+    std::lock_guard<std::mutex> lk(capture_lock());
+    t->mv_variant = matvec_variant;
+    t->rb_threads = sweep_threads;
+    t->drop_step_graphs();   // captured with the old launch shapes
+    if (t->pg_exec) { HIPCHK(hipGraphExecDestroy(t->pg_exec)); t->pg_exec = nullptr; }
+    return 0;
+}
+
+// Multigrid hierarchy ow3d_create would build for cfg (host only): level count and coarsest unknowns.
+extern "C" int ow3d_mg_plan(const ow3d_config *cfg, int *nlev, int *coarsest) {
+    if (!cfg || cfg->nx < 2 || cfg->ny < 1 || cfg->nz < 3) { err() = "invalid configuration"; return -1; }
+    Grid g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz; g.gx = 1; g.gy = cfg->ny > 1 ? 1 : 0;
+    g.a = cfg->alpha; g.b = cfg->ny > 1 ? cfg->beta : 0; g.c = cfg->gamma;
+    g.alloc();
+    std::vector<Grid> lev;
+    if (!mg_levels(g, cfg->mg_maxgrids, lev, err())) return -1;
+    if (nlev) *nlev = (int)lev.size();
+    if (coarsest) *coarsest = (int)lev[0].n;
+    return 0;
+}
 
 // Device memory ow3d_create would allocate for cfg (the buffers it allocates, without the allocator's
 // rounding): fine-grid vectors and Krylov basis, surface planes, operator tables, multigrid levels.

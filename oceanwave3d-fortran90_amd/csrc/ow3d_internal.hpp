@@ -112,6 +112,10 @@ struct ow3d_tank {
     std::vector<std::unique_ptr<MgLevel<float>>> mg32;
     int bot_top = -1;                 // finest level handled by the one-workgroup bottom kernel (-1: none)
     bool fused_rb = true;             // one launch per zebra sweep (red and black halves) on the upper levels
+    // launch choices (ow3d_tune): matvec variant 0 column kernel, widest tile / 1 column kernel, 8-row
+    // tiles / 2 point kernel; zebra-sweep block size 0 auto (1024 threads below 256 tiles) / 256 / 1024
+    int mv_variant = 0;
+    int rb_threads = 0;
     BottomArgs<double> bot64{};
     BottomArgs<float> bot32{};
     DevBuf<BottomArgs<double>> bot64d;   // device copies read by the bottom kernel

@@ -698,6 +698,15 @@ __global__ void k_convert(size_t n, const S *__restrict__ x, D *__restrict__ y, 
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         y[q] = (D)x[q];
 }
+// Preconditioner input: b = (T) r and the iterate x = 0 in one pass.
+template <typename T>
+__global__ void k_precond_in(size_t n, const double *__restrict__ r, T *__restrict__ b, T *__restrict__ x, const int *skip) {
+    if (solve_done(skip)) return;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        b[q] = (T)r[q];
+        x[q] = T(0);
+    }
+}
 template <typename T>
 __global__ void k_fill(size_t n, T v, T *__restrict__ y, const int *skip) {
     if (solve_done(skip)) return;
@@ -876,27 +885,28 @@ static __global__ void k_gmres_start(KrylovState *S, const double *hv, double rt
     S->g[0] = beta;
 }
 
-// Hessenberg column k (1-based step) from its projections col[0..k-1] and the new vector's norm hn;
-// previous rotations, the new rotation, the residual estimate and the stopping test.
-__device__ inline void gmres_column(KrylovState *S, int k, const double *col, double hn, int m) {
+// Hessenberg column k (1-based step): col[0..k-1] projections, col[k] = hn the new vector's norm, held in
+// LDS with the previous rotations (cs, sn).  One lane applies the previous rotations and the new one and
+// updates the residual estimate and the stopping test; the caller writes col back as column k - 1 of H.
+__device__ inline void gmres_column(KrylovState *S, int k, double *col, const double *cs, const double *sn, int m) {
     constexpr int M = KRYLOV_MAXK;
-    double *H = S->H;
-    for (int i = 0; i < k; i++) H[i * M + (k - 1)] = col[i];
-    H[k * M + (k - 1)] = hn;
+    const double hn = col[k];
     for (int i = 0; i < k - 1; i++) {
-        const double a = H[i * M + (k - 1)], b = H[(i + 1) * M + (k - 1)];
-        H[i * M + (k - 1)] = S->cs[i] * a + S->sn[i] * b;
-        H[(i + 1) * M + (k - 1)] = S->cs[i] * b - S->sn[i] * a;
+        const double a = col[i], b = col[i + 1];
+        col[i] = cs[i] * a + sn[i] * b;
+        col[i + 1] = cs[i] * b - sn[i] * a;
     }
-    const double a = H[(k - 1) * M + (k - 1)], b = H[k * M + (k - 1)];
+    const double a = col[k - 1], b = col[k];
     const double rr = sqrt(a * a + b * b);
     double c = 1.0, sv = 0.0;
     if (rr > 0.0) { c = a / rr; sv = b / rr; }
     S->cs[k - 1] = c; S->sn[k - 1] = sv;
-    H[(k - 1) * M + (k - 1)] = rr;
-    S->g[k] = -sv * S->g[k - 1];
-    S->g[k - 1] = c * S->g[k - 1];
-    const double est = fabs(S->g[k]);
+    col[k - 1] = rr;
+    col[k] = hn;
+    const double g0 = S->g[k - 1];
+    S->g[k] = -sv * g0;
+    S->g[k - 1] = c * g0;
+    const double est = fabs(-sv * g0);
     S->est = est; S->k = k;
     S->scale = hn > 0.0 ? 1.0 / hn : 1.0;
     if (!(k < m && !(est <= S->tol) && S->matvecs < m) || hn == 0.0 || k + 1 >= M) S->done = 1;
@@ -904,28 +914,55 @@ __device__ inline void gmres_column(KrylovState *S, int k, const double *col, do
 
 // Arnoldi step k, first pass: hv[0..k-1] = V_i . w, hv[k] = w . w (classical Gram-Schmidt).  Keeps the
 // column when the projection removed at most half of the squared norm, else asks for the second pass.
-static __global__ void k_gmres_pass1(KrylovState *S, const double *hv, int k, int m) {
-    if (threadIdx.x != 0 || S->done) return;
-    S->matvecs += 1;
-    const double nrm2 = hv[k];
-    double nn = nrm2;
-    for (int i = 0; i < k; i++) nn -= hv[i] * hv[i];
-    if (nn > 0.5 * nrm2) {
-        gmres_column(S, k, hv, sqrt(nn), m);
-        S->skip2 = 1;
-    } else {
-        for (int i = 0; i < k; i++) S->h1[i] = hv[i];
-        S->skip2 = 0;
+// One wave: the vectors are staged in LDS by all lanes, the serial part runs on lane 0.
+static __global__ __launch_bounds__(64) void k_gmres_pass1(KrylovState *S, const double *hv, int k, int m) {
+    constexpr int M = KRYLOV_MAXK;
+    __shared__ double col[M + 1], cs[M], sn[M];
+    __shared__ int second;
+    if (S->done) return;
+    const int t = threadIdx.x;
+    if (t <= k) col[t] = hv[t];
+    if (t < k - 1) { cs[t] = S->cs[t]; sn[t] = S->sn[t]; }
+    __syncthreads();
+    if (t == 0) {
+        S->matvecs += 1;
+        const double nrm2 = col[k];
+        double nn = nrm2;
+        for (int i = 0; i < k; i++) nn -= col[i] * col[i];
+        second = !(nn > 0.5 * nrm2);
+        if (!second) {
+            col[k] = sqrt(nn);
+            gmres_column(S, k, col, cs, sn, m);
+        }
+        S->skip2 = second ? 0 : 1;
+    }
+    __syncthreads();
+    if (second) {
+        if (t < k) S->h1[t] = col[t];
+    } else if (t <= k) {
+        S->H[t * M + (k - 1)] = col[t];
     }
 }
 
 // Arnoldi step k, second (DGKS) pass: hv holds the projections of the once-orthogonalised vector.
-static __global__ void k_gmres_pass2(KrylovState *S, const double *hv, int k, int m) {
-    if (threadIdx.x != 0 || S->skip2) return;
-    double n2 = hv[k], col[KRYLOV_MAXK];
-    for (int i = 0; i < k; i++) { n2 -= hv[i] * hv[i]; col[i] = S->h1[i] + hv[i]; }
-    gmres_column(S, k, col, sqrt(fmax(n2, 0.0)), m);
-    S->skip2 = 1;
+static __global__ __launch_bounds__(64) void k_gmres_pass2(KrylovState *S, const double *hv, int k, int m) {
+    constexpr int M = KRYLOV_MAXK;
+    __shared__ double col[M + 1], h2[M + 1], cs[M], sn[M];
+    if (S->skip2) return;
+    const int t = threadIdx.x;
+    if (t <= k) h2[t] = hv[t];
+    if (t < k) col[t] = S->h1[t];
+    if (t < k - 1) { cs[t] = S->cs[t]; sn[t] = S->sn[t]; }
+    __syncthreads();
+    if (t == 0) {
+        double n2 = h2[k];
+        for (int i = 0; i < k; i++) { n2 -= h2[i] * h2[i]; col[i] = col[i] + h2[i]; }
+        col[k] = sqrt(fmax(n2, 0.0));
+        gmres_column(S, k, col, cs, sn, m);
+        S->skip2 = 1;
+    }
+    __syncthreads();
+    if (t <= k) S->H[t * M + (k - 1)] = col[t];
 }
 
 __device__ inline void solve_stats(KrylovState *S) {

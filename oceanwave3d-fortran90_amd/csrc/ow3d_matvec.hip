@@ -21,23 +21,30 @@ constexpr size_t mv3c_lds() {
     return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA + 7 * 16 * TY) * 8;
 }
 template <int NZ, int A>
-int launch_mv3c(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
-    constexpr int TY = mv3c_lds<NZ, A, 16>() <= 65536 ? 16 : 8;
-    static_assert(mv3c_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
-    const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, a, phi, y);
-    return 0;
-}
-template <int NZ, int A>
-int launch_mv3(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
-    static const bool col = [] { const char *e = std::getenv("OW3D_MV"); return !(e && e[0] == 'p'); }();
-    if (col) return launch_mv3c<NZ, A>(t, a, phi, y, s);
+int launch_mv3p(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     constexpr int TY = mv3_lds<NZ, A, 8>() <= 65536 ? 8 : 4;
     static_assert(mv3_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
     const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
     hipLaunchKernelGGL((k_matvec3p<NZ, A, TY, 256>), dim3(nb), dim3(256), 0, s, a, phi, y);
     return 0;
 }
+template <int NZ, int A, int TY>
+int launch_mv3c_ty(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
+    static_assert(mv3c_lds<NZ, A, TY>() <= 65536, "matvec tile exceeds 64 KiB of LDS");
+    const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
+    hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, a, phi, y);
+    return 0;
+}
+// matvec variant (ow3d_tune): 0 column kernel with the widest tile that fits 64 KiB of LDS, 1 column kernel
+// with 16 x 8 tiles, 2 point kernel (k_matvec3p)
+template <int NZ, int A>
+int launch_mv3(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
+    constexpr int TY = mv3c_lds<NZ, A, 16>() <= 65536 ? 16 : 8;
+    if (t->mv_variant == 0) return launch_mv3c_ty<NZ, A, TY>(t, a, phi, y, s);
+    if (t->mv_variant == 1) return launch_mv3c_ty<NZ, A, 8>(t, a, phi, y, s);
+    return launch_mv3p<NZ, A>(t, a, phi, y, s);
+}
+
 template <int NZ, int A>
 int launch_mv2(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
     dim3 grid((t->g.nxg + 63) / 64);

@@ -272,7 +272,7 @@ void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
 // Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
 // under it while other blocks write their results).
 template <typename T, int NZ>
-void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
+void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s) {
     // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
@@ -281,14 +281,15 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStre
     dim3 grid((nxi + offx + TX - 1) / TX, L.dim3 ? (nyi + offy + TY - 1) / TY : 1);
     // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
-    if ((long)grid.x * grid.y < 256) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b);
+    const bool big = threads == 1024 || (threads == 0 && (long)grid.x * grid.y < 256);
+    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b);
     else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b);
 }
 
 template <typename T>
-int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s) {
+int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s) {
     switch (L.nzg) {
-#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, s); return 0;
+#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s); return 0;
         OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
         OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
 #undef OW3D_NZ
@@ -304,7 +305,7 @@ int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s) {
         if (nu == 0) ghost_pass(L, cur, s);
         for (int it = 0; it < nu; it++) {
             T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
-            if (rb_launch<T>(L.args, cur, nxt, L.b.p, s)) return -1;
+            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s)) return -1;
             cur = nxt;
         }
         return 0;
@@ -363,8 +364,7 @@ int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, con
     const size_t n = t->g.n;
     const int top = t->nlev - 1;
     MgLevel<T> &F = *mg[top];
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, t->done());
-    hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, T(0), F.x.p, t->done());
+    hipLaunchKernelGGL((k_precond_in<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, F.x.p, t->done());
     T *cur = F.x.p;
     for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
         if (t->nlev == 1 && it > 0) break;

@@ -53,7 +53,7 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
            "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
-           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats"]
+           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan"]
 
 
 class OW3DError(RuntimeError):
@@ -97,6 +97,8 @@ def lib():
         L.ow3d_wait.argtypes = [P]
         L.ow3d_device_bytes.restype = ctypes.c_longlong
         L.ow3d_device_bytes.argtypes = [P]
+        L.ow3d_tune.argtypes = [P, I, I]
+        L.ow3d_mg_plan.argtypes = [ctypes.POINTER(Config), IP, IP]
         L.ow3d_step_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), D]
         L.ow3d_estimate_bytes.restype = ctypes.c_longlong
         L.ow3d_estimate_bytes.argtypes = [ctypes.POINTER(Config)]
@@ -244,6 +246,10 @@ class Tank:
     def replays(self):
         return self.step_stats()[0]
 
+    def tune(self, matvec_variant=0, sweep_threads=0):
+        """Launch choices (see ow3d_tune in include/ow3d.h); results do not depend on them."""
+        _check(lib().ow3d_tune(self.h, int(matvec_variant), int(sweep_threads)), "ow3d_tune")
+
 
 def estimate_bytes(cfg):
     """Device memory Tank(cfg, ...) would allocate (before the allocator's rounding)."""
@@ -253,6 +259,16 @@ def estimate_bytes(cfg):
     if v < 0:
         raise OW3DError(f"ow3d_estimate_bytes: {lib().ow3d_last_error().decode()}")
     return int(v)
+
+
+def mg_plan(cfg):
+    """(levels, coarsest unknowns) of the multigrid hierarchy Tank(cfg, ...) would build (host only)."""
+    c = dict(DEFAULTS)
+    c.update(cfg)
+    nl = ctypes.c_int(); nc = ctypes.c_int()
+    if lib().ow3d_mg_plan(ctypes.byref(Config(**c)), ctypes.byref(nl), ctypes.byref(nc)) < 0:
+        raise OW3DError(f"ow3d_mg_plan: {lib().ow3d_last_error().decode()}")
+    return nl.value, nc.value
 
 
 def batch_rk4_steps(tanks, nsteps, dts):

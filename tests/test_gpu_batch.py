@@ -188,3 +188,26 @@ def test_request_loop_respects_the_memory_budget():
     out = serve.Server(hbm_budget=budget, max_active=8).run(reqs)
     assert out["completed"] == 4
     assert out["queue_delay_s"]["p99"] > 0.0
+
+
+def test_launch_choices_do_not_change_results():
+    """Zebra-sweep block size only changes the thread mapping (bit-identical); the point matvec kernel
+    sums the same terms in another order (within 1e-11)."""
+    from ow3d import problems
+    spec = problems.batch_tank(71, sizes=(65,), nzs=(9,), orders=(6,), bottom="shoal")
+    ref, dt = make(spec)
+    ref.rk4_steps(4, 0.0, dt)
+    Eref = ref.get_state()
+    for mv, thr, exact in ((0, 256, True), (0, 1024, True), (1, 0, False), (2, 0, False)):
+        T, _ = make(spec)
+        T.tune(mv, thr)
+        T.rk4_steps(4, 0.0, dt)
+        got = T.get_state()
+        if exact:
+            assert same(got, Eref), (mv, thr)
+        else:
+            for a, b in zip(got, Eref):
+                assert np.max(np.abs(a - b)) <= 1e-11 * max(np.max(np.abs(b)), 1e-300), (mv, thr)
+        T.close()
+    with pytest.raises(Exception):
+        ref.tune(3, 0)

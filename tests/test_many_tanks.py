@@ -62,3 +62,24 @@ def test_even_columns_are_kept_while_the_plane_is_coarsened(oracle_mod):
         T = oracle_mod.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
         got = [(a - 2, b - 2, c - 1) for (a, b, c) in (T.level_dims(k) for k in range(T.nlev - 1, -1, -1))]
         assert got[:len(want)] == want, (nx, ny, nz, got)
+
+
+def test_autotuner_candidates_and_cache(tmp_path):
+    import ow3d
+    from ow3d import autotune
+    for cfg in (dict(nx=513, ny=513, nz=9), dict(nx=65, ny=129, nz=12), dict(nx=129, ny=1, nz=9)):
+        cands = autotune.level_candidates(cfg)
+        assert cands == sorted(cands) and 1 <= len(cands) <= 3
+        for L in cands:
+            n, nc = ow3d.mg_plan(dict(cfg, mg_maxgrids=L))
+            assert n == L and nc <= autotune.MAX_COARSE
+        # one level fewer than the first candidate would exceed the dense-solve limit
+        if cands[0] > 2:
+            assert ow3d.mg_plan(dict(cfg, mg_maxgrids=cands[0] - 1))[1] > autotune.MAX_COARSE
+    assert ow3d.mg_plan(dict(nx=513, ny=513, nz=9, mg_maxgrids=7)) == (7, 11 * 11 * 10)   # 9^2 x 9 coarsest
+    path = str(tmp_path / "t.json")
+    table = {autotune.class_key(dict(nx=65, ny=65, nz=9, alpha=2)): {"mg_maxgrids": 4, "matvec_variant": 1, "sweep_threads": 0}}
+    autotune.save(table, path)
+    back = autotune.load(path)
+    assert autotune.tuned(dict(nx=65, ny=65, nz=9, alpha=2), back)["matvec_variant"] == 1
+    assert autotune.tuned(dict(nx=65, ny=65, nz=9, alpha=3), back) is None
