@@ -252,11 +252,10 @@ template <int NZ, int A, int TY>
 __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
     if (solve_done(g.skip)) return;
     constexpr int TX = 16, NT = TX * TY, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
-    constexpr int PS = (WX * WY) | 1, NC = TX * TY;
+    constexpr int PS = (WX * WY) | 1;
     __shared__ double sP[NZ * PS];
     __shared__ double sSX[(TX + 2) * 2 * RA];
     __shared__ double sSY[(TY + 2) * 2 * RA];
-    __shared__ double sCF[7 * NC];
     const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
     const int gxt = (nxg + TX - 1) / TX;
     const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -265,21 +264,15 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
     const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
     {
-        // every HBM load of the block first: the phi window (stored level-major), the coefficient planes
-        // of the tile and the stencil rows of its columns and rows
-        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT + NT - 1) / NT, CPT = (7 * NC + NT - 1) / NT;
-        double v[PER], cv[CPT];
+        // every HBM load of the block first: the phi window (stored level-major) and the stencil rows of
+        // its columns and rows (each thread reads its own column's coefficient planes where it uses them)
+        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT + NT - 1) / NT;
+        double v[PER];
         const double *src = phi + ((size_t)loy * nxg + lox) * NZ;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
             v[q] = (e < TOT && r < wy && c < wx * NZ) ? src[(size_t)r * nxg * NZ + c] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int e = (int)threadIdx.x + q * NT;
-            const int pl = e / NC, c = e - pl * NC, ii = i0 + c % TX, jj = j0 + c / TX;
-            cv[q] = (e < 7 * NC && ii < nxg && jj < nyg) ? g.cf[(size_t)pl * nxy + ii + (size_t)nxg * jj] : 0.0;
         }
         for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
             const int r = e / (2 * RA), ii = i0 - 1 + r;
@@ -297,17 +290,12 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
                 sP[k * PS + r * WX + cc] = v[q];
             }
         }
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int e = (int)threadIdx.x + q * NT;
-            if (e < 7 * NC) sCF[e] = cv[q];
-        }
     }
     __syncthreads();
     auto P = [&](int k, int ii, int jj) -> double { return sP[k * PS + (jj - loy) * WX + (ii - lox)]; };
     auto SX = [&](int ii) -> const double * { return sSX + (ii - (i0 - 1)) * 2 * RA; };
     auto SY = [&](int jj) -> const double * { return sSY + (jj - (j0 - 1)) * 2 * RA; };
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX, col = threadIdx.x;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = (i < nxg) && (j < nyg);
     double o[NZ];
@@ -378,8 +366,9 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
                     dyc0 = t;
                 }
             }
-            const double A2 = sCF[col], B2 = sCF[NC + col], A4 = sCF[2 * NC + col], B4 = sCF[3 * NC + col];
-            const double A3 = sCF[4 * NC + col], B3 = sCF[5 * NC + col], S5 = sCF[6 * NC + col];
+            const double *cfp = g.cf + i + (size_t)nxg * j;
+            const double A2 = cfp[0], B2 = cfp[nxy], A4 = cfp[2 * (size_t)nxy], B4 = cfp[3 * (size_t)nxy];
+            const double A3 = cfp[4 * (size_t)nxy], B3 = cfp[5 * (size_t)nxy], S5 = cfp[6 * (size_t)nxy];
             double obot;
             {
                 constexpr int wz = (1 - A) < 0 ? 0 : (1 - A);
@@ -1776,28 +1765,34 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
             i = ib + (((ib + j) & 1) != COL ? 1 : 0) + 2 * t;
             return i <= iend && j >= jlo && j < jhi && i >= L.gx && i < nxg - L.gx;
         };
-        for (int task = threadIdx.x; task < ns * NZ; task += NT) {
-            const int sl = task / NZ, k = task - sl * NZ;
+        // rows in level-major order (task = k * NSM + slot): the lanes of a wave share the level, so
+        // bottom, interior and surface rows do not diverge inside a wave; row arrays are slot-fastest
+        // (sR[(m * NZ + k) * NS + slot]), conflict-free for both phases
+        constexpr int NSM = PER * (COL == 0 ? TY + 2 : TY);
+        for (int task = threadIdx.x; task < NSM * NZ; task += NT) {
+            const int k = task / NSM, sl = task - k * NSM;
             int i, j;
-            if (!slot(sl, i, j)) continue;
+            if (sl >= ns || !slot(sl, i, j)) continue;
             T lo, di, up, rh, e2;
             const T bk = sB[((j - bloy) * (TX + 2) + (i - blox)) * NZ + k];
             if (i >= 2 && i <= nxg - 3 && (!d3 || (j >= 2 && j <= nyg - 3)))
                 zebra_row_interior<T, NZP, WX * NZP>(C, d3, i, j, k, NZ, sX + ((j - loy) * WX + (i - lox)) * NZP, bk, lo, di, up, rh, e2);
             else
                 zebra_row<T>(L, C, i, j, k, NZ, X, bk, lo, di, up, rh, e2);
-            T *R = sR + (size_t)sl * 4 * NZ;
-            R[k] = lo; R[NZ + k] = di; R[2 * NZ + k] = up; R[3 * NZ + k] = rh;
+            sR[(0 * NZ + k) * NS + sl] = lo; sR[(1 * NZ + k) * NS + sl] = di;
+            sR[(2 * NZ + k) * NS + sl] = up; sR[(3 * NZ + k) * NS + sl] = rh;
             if (k == 0) sE[sl] = e2;
         }
         __syncthreads();
         for (int sl = threadIdx.x; sl < ns; sl += NT) {
             int i, j;
             if (!slot(sl, i, j)) continue;
-            const T *R = sR + (size_t)sl * 4 * NZ;
             T lo[NZ], di[NZ], up[NZ], rh[NZ];
 #pragma unroll
-            for (int k = 0; k < NZ; k++) { lo[k] = R[k]; di[k] = R[NZ + k]; up[k] = R[2 * NZ + k]; rh[k] = R[3 * NZ + k]; }
+            for (int k = 0; k < NZ; k++) {
+                lo[k] = sR[(0 * NZ + k) * NS + sl]; di[k] = sR[(1 * NZ + k) * NS + sl];
+                up[k] = sR[(2 * NZ + k) * NS + sl]; rh[k] = sR[(3 * NZ + k) * NS + sl];
+            }
             zebra_thomas<T, NZ>(lo, di, up, rh, sE[sl], sX + ((j - loy) * WX + (i - lox)) * NZP);
         }
         __syncthreads();

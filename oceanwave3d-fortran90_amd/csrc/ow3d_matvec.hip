@@ -18,7 +18,7 @@ constexpr size_t mv3_lds() {
 template <int NZ, int A, int TY>
 constexpr size_t mv3c_lds() {
     constexpr int R = A + 1, RA = 2 * A + 1;
-    return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA + 7 * 16 * TY) * 8;
+    return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA) * 8;
 }
 template <int NZ, int A>
 int launch_mv3p(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
