@@ -40,7 +40,7 @@ typedef struct ow3d_config {
     int nu1, nu2;              /* pre / post smoothing sweeps */
     int mg_maxgrids;           /* cap on multigrid levels (<= 0: no cap) */
     int extrapolation;         /* extrapolationONOFF: initial guesses from earlier solutions */
-    int mg_precision;          /* 0 FP64 multigrid, 1 FP32 multigrid, 2 FP32 + FP16 storage on the two coarsest levels */
+    int mg_precision;          /* 0 FP64 multigrid, 1 FP32 multigrid, 2 FP32 multigrid with the coarsest level's dense inverse stored in FP16 */
 } ow3d_config;
 
 /* Replaces the set-up the reference does in OceanWave3DT0Setup.f90:32-409 for this path

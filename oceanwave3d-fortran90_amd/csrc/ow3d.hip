@@ -195,7 +195,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (c.ny > 1 && (c.alpha != c.beta || c.alpha != c.gamma)) { err() = "alpha, beta, gamma must be equal"; return -1; }
     if (c.ny == 1 && c.alpha != c.gamma) { err() = "alpha and gamma must be equal"; return -1; }
     if (c.gmres_m >= MAXK) { err() = "GMRESmaxiterations must be < 32"; return -1; }
-    if (c.mg_precision < 0 || c.mg_precision > 1) { err() = "mg_precision must be 0 (FP64) or 1 (FP32)"; return -1; }
+    if (c.mg_precision < 0 || c.mg_precision > 2) { err() = "mg_precision must be 0 (FP64), 1 (FP32) or 2 (FP32, FP16 coarsest inverse)"; return -1; }
     if (c.precond != 0 && c.precond != 3) { err() = "precond must be 0 or 3"; return -1; }
     auto t = std::make_unique<ow3d_tank>();
     t->cfg = c;
@@ -770,11 +770,11 @@ extern "C" long long ow3d_estimate_bytes(const ow3d_config *cfg) {
     if (cfg->precond == 3) {
         std::vector<Grid> lev;
         if (!mg_levels(g, cfg->mg_maxgrids, lev, err())) return -1;
-        const long long p = cfg->mg_precision == 0 ? 8 : 4;
+        const long long p = cfg->mg_precision == 0 ? 8 : 4, pc = cfg->mg_precision == 2 ? 2 : p;
         for (size_t k = 0; k < lev.size(); k++) {
             const Grid &L = lev[k];
             b += p * ((long long)L.n * 4 + 6LL * L.nxy + 12LL * (L.nxg + L.nyg + L.nzg)) + 8LL * (L.nxg + L.nyg + L.nzg);
-            if (k == 0) b += p * (long long)L.n * (long long)L.n;
+            if (k == 0) b += pc * (long long)L.n * (long long)L.n;
         }
     }
     return b;

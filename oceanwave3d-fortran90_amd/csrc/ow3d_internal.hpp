@@ -1,6 +1,7 @@
 // Declarations shared by the host translation units of libow3d (ow3d.hip: C ABI, Krylov solvers and
 // RK4; ow3d_matvec.hip: high-order operator dispatch; ow3d_mg.hip: multigrid set-up and V cycle).
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,6 +74,7 @@ template <typename T>
 struct MgLevel {
     Grid g;
     DevBuf<T> sx, sy, sz, zs, col, x, x2, b, r, minv;   // x2: second iterate buffer of out-of-place sweeps
+    DevBuf<__half> minv16;                             // coarsest-level inverse in FP16 (mg_precision 2)
     DevBuf<double> xf, yf, zf;   // level coordinates (restriction / prolongation weights)
     LvArgs<T> args{};
 };

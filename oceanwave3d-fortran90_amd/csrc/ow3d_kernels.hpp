@@ -1980,17 +1980,18 @@ __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__r
     prolong_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, ec, xf);
 }
 
-// Coarsest level: x = Minv b with the dense inverse computed at set-up.
-template <typename T>
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, const T *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x,
+// Coarsest level: x = Minv b with the dense inverse computed at set-up, stored as M (T, or FP16 with
+// sums in T).
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, const M *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x,
                                                    const int *skip) {
     if (solve_done(skip)) return;
     // one 64-lane wave per row: coalesced reads of the row, fixed-order lane sums and wave reduction
     const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n) return;
-    const T *row = Minv + (size_t)r * n;
+    const M *row = Minv + (size_t)r * n;
     T s = T(0);
-    for (int c = lane; c < n; c += 64) s += row[c] * b[c];
+    for (int c = lane; c < n; c += 64) s += (T)row[c] * b[c];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) x[r] = s;

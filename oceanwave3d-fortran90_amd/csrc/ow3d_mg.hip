@@ -20,6 +20,7 @@ int mg_setup(ow3d_tank *t) {
     // The group must share one column depth on levels 1 .. bot_top (level 0 is solved densely).
     long lim = 0;
     if (const char *e = std::getenv("OW3D_MG_BOTTOM")) lim = std::atol(e);
+    if (c.mg_precision == 2) lim = 0;   // the bottom kernel reads the FP32 inverse
     if (const char *e = std::getenv("OW3D_MG_FUSED_RB")) t->fused_rb = e[0] != '0';
     t->bot_top = -1;
     for (int k = 1; k < t->nlev && k < MG_BOTTOM_MAX; k++) {

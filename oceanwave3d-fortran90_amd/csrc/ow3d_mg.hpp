@@ -221,7 +221,14 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
             }
             std::vector<double> Ainv;
             dense_inverse(A, n, Ainv);
-            HIPCHK(up(L->minv, Ainv));
+            if (t->cfg.mg_precision == 2) {   // FP16 storage of the coarsest level's inverse
+                std::vector<__half> h(Ainv.size());
+                for (size_t q = 0; q < Ainv.size(); q++) h[q] = __float2half((float)Ainv[q]);
+                HIPCHK(L->minv16.alloc(h.size()));
+                HIPCHK(hipMemcpy(L->minv16.p, h.data(), h.size() * sizeof(__half), hipMemcpyHostToDevice));
+            } else {
+                HIPCHK(up(L->minv, Ainv));
+            }
         }
         L->args.nxg = g.nxg; L->args.nyg = g.nyg; L->args.nzg = g.nzg; L->args.nxy = g.nxy;
         L->args.gx = g.gx; L->args.gy = g.gy; L->args.dim3 = g.ny > 1;
@@ -340,7 +347,12 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
         return 0;
     }
     if (k == 0) {
-        hipLaunchKernelGGL((k_mg_coarse<T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p, (const T *)L.b.p, L.x.p, t->done());
+        if (L.minv16.p)
+            hipLaunchKernelGGL((k_mg_coarse<T, __half>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const __half *)L.minv16.p,
+                               (const T *)L.b.p, L.x.p, t->done());
+        else
+            hipLaunchKernelGGL((k_mg_coarse<T, T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p,
+                               (const T *)L.b.p, L.x.p, t->done());
         cur = L.x.p;
         return 0;
     }

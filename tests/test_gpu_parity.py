@@ -190,3 +190,20 @@ def test_rk4_parity(oracle_mod, nx, ny, nz, shoal, sponge):
     assert rel(Eg, Eo) < 1e-10, rel(Eg, Eo)
     assert rel(Pg, Po) < 1e-10, rel(Pg, Po)
     assert rel(Wg, Wo) < 1e-9
+
+
+def test_fp16_coarse_storage(oracle_mod):
+    """mg_precision 2: the FP32 V cycle with the coarsest inverse in FP16 stays within FP16 rounding of
+    the FP32 one, and RK4 steps with it stay within 1e-6 of FP64-preconditioned steps."""
+    from ow3d import problems
+    T_o, T32, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=1, mg_maxgrids=3)
+    _, T16, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=2, mg_maxgrids=3)
+    _, T64, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=0, mg_maxgrids=3)
+    r = np.random.default_rng(9).standard_normal(T_o.shape3)
+    u32, u16 = T32.precond_apply(r), T16.precond_apply(r)
+    assert 0 < rel(u16, u32) < 2e-3
+    E, P = surface(33, 33, 10.0, seed=3, amp=0.01)
+    for T in (T16, T64):
+        T.set_state(E, 0.01 * P)
+        assert T.rk4_steps(4, 0.0, 0.05) == 0
+    assert rel(T16.get_state()[0], T64.get_state()[0]) < 1e-6
