@@ -95,6 +95,11 @@ int ow3d_rk4_steps(ow3d_tank *t, int nsteps, double time0, double dt);
 /* iteration statistics as in iterative_solution.f90:117-134: total, solves, min, max */
 int ow3d_stats(const ow3d_tank *t, long long *total, long long *solves, int *itmin, int *itmax);
 
+/* Volume kinematics: replaces the velocity part of StoreKinematicData (IO/StoreKinematicData.f90:266-326).
+ * U, V = horizontal velocities, W = vertical velocity at every grid point (3D layout) from phi (NULL: the
+ * handle's current solution) and the handle's current surface. */
+int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, double *V, double *W);
+
 /* Device-pointer form of ow3d_laplace_apply for callers that keep phi resident (stream: hipStream_t
  * or NULL for the handle's stream). */
 int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, double *d_y, void *stream);

@@ -240,6 +240,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (c.ny == 1) sy.assign(6, 0.0);
     std::vector<double> zs(g.z); zs[0] = 0.0;
     HIPCHK(upd(t->sx, g.sx)); HIPCHK(upd(t->sy, sy)); HIPCHK(upd(t->sz, g.sz)); HIPCHK(upd(t->zs, zs)); HIPCHK(upd(t->h, g.h));
+    HIPCHK(upd(t->zr, g.z));
     std::vector<double> bath((size_t)3 * nxy);
     for (size_t p = 0; p < nxy; p++) {
         bath[p] = g.hx[p]; bath[nxy + p] = c.ny > 1 ? g.hy[p] : 0.0; bath[2 * nxy + p] = g.hxx[p] + (c.ny > 1 ? g.hyy[p] : 0.0);
@@ -725,6 +726,32 @@ extern "C" int ow3d_stats(const ow3d_tank *t, long long *total, long long *solve
 }
 
 extern "C" long long ow3d_device_bytes(const ow3d_tank *t) { return t ? t->bytes : -1; }
+
+extern "C" int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, double *V, double *W) {
+    if (!t || !U || !V || !W) { err() = "null argument"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    std::lock_guard<std::mutex> lk(capture_lock());
+    hipStream_t s = t->s;
+    const size_t n = t->g.n;
+    const double *src = t->phi.p;
+    if (phi) {
+        HIPCHK(hipMemcpyAsync(t->io_a.p, phi, n * 8, hipMemcpyHostToDevice, s));
+        src = t->io_a.p;
+    }
+    KinArgs a{};
+    a.nxg = t->g.nxg; a.nyg = t->g.nyg; a.nzg = t->g.nzg; a.a = t->g.a; a.b = t->g.b; a.c = t->g.c;
+    a.nonlinear = t->cfg.nonlinear;
+    a.sx = t->sx.p; a.sy = t->sy.p; a.sz = t->sz.p; a.z = t->zr.p; a.h = t->h.p; a.bath = t->bath.p;
+    a.E = t->E.p; a.Ex = t->Ex.p; a.Ey = t->Ey.p;
+    // outputs: U in io_b, V in w, W in z (scratch vectors of the handle, free outside a solve)
+    hipLaunchKernelGGL(k_kinematics, dim3(grid_blocks(n)), dim3(256), 0, s, a, src, t->io_b.p, t->w.p, t->z.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(U, t->io_b.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(V, t->w.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(W, t->z.p, n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
 
 extern "C" int ow3d_tune(ow3d_tank *t, int matvec_variant, int sweep_threads) {
     if (!t) { err() = "null handle"; return -1; }

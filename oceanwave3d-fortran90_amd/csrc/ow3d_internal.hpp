@@ -95,7 +95,7 @@ struct ow3d_tank {
     hipStream_t s = nullptr;
     int A = 3;
     // high-order operator
-    DevBuf<double> sx, sy, sz, zs, bath, h, cf;
+    DevBuf<double> sx, sy, sz, zs, zr, bath, h, cf;   // zr: sigma coordinates with the ghost level's
     MvArgs mv{};
     // surface state
     DevBuf<double> E, P, W, Ex, Exx, Ey, Eyy, Px, Py, Et, Pt, Wt, KE, KP;

@@ -651,6 +651,49 @@ static __global__ void k_stage(StageArgs a) {
     a.Eo[p] = ne; a.Po[p] = np_;
 }
 
+// Fluid velocities at every grid point from phi (StoreKinematicData.f90:266-326): W = dphi/dsigma, U =
+// dphi/dx and V = dphi/dy with the path's first-derivative stencil rows (off-centred near the ends as
+// DiffZArbitrary / DiffXEven / DiffYEven), in nonlinear mode plus the chain-rule terms
+// ((1 - z)/d h_x - z/d eta_x) W and ((1 - z)/d h_y - z/d eta_y) W (z: the level's sigma coordinate, the
+// ghost level's included; d = h + eta); the vertical velocity is W / d.  One thread per point.
+struct KinArgs {
+    int nxg, nyg, nzg, a, b, c, nonlinear;
+    const double *sx, *sy, *sz;    // stencil rows [N][2][2a+1] (first derivative first)
+    const double *z;               // sigma coordinate of each level, ghost level included
+    const double *h, *bath;        // depth; bath planes h_x, h_y
+    const double *E, *Ex, *Ey;     // surface and its derivatives
+};
+static __global__ __launch_bounds__(256) void k_kinematics(KinArgs a, const double *__restrict__ phi, double *__restrict__ U,
+                                                           double *__restrict__ V, double *__restrict__ Wv) {
+    const size_t nxy = (size_t)a.nxg * a.nyg, n = nxy * a.nzg;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q % a.nzg);
+        const size_t p = q / a.nzg;
+        const int i = (int)(p % a.nxg), j = (int)(p / a.nxg);
+        const int rz = 2 * a.c + 1, wz = cclamp(k - a.c, 0, a.nzg - rz);
+        const double *Sz = a.sz + (size_t)(k * 2) * rz;
+        double w = 0.0;
+        for (int m = 0; m < rz; m++) w += Sz[m] * phi[p * a.nzg + wz + m];
+        double u = 0.0, v = 0.0;
+        if (a.nxg > 1) {
+            const int rx = 2 * a.a + 1, wx = cclamp(i - a.a, 0, a.nxg - rx);
+            const double *Sx = a.sx + (size_t)(i * 2) * rx;
+            for (int m = 0; m < rx; m++) u += Sx[m] * phi[((size_t)j * a.nxg + wx + m) * a.nzg + k];
+        }
+        if (a.nyg > 1) {
+            const int ry = 2 * a.b + 1, wy = cclamp(j - a.b, 0, a.nyg - ry);
+            const double *Sy = a.sy + (size_t)(j * 2) * ry;
+            for (int m = 0; m < ry; m++) v += Sy[m] * phi[((size_t)(wy + m) * a.nxg + i) * a.nzg + k];
+        }
+        const double d = a.h[p] + a.E[p], z = a.z[k];
+        if (a.nonlinear) {
+            if (a.nxg > 1) u = u + ((1 - z) / d * a.bath[p] - z / d * a.Ex[p]) * w;
+            if (a.nyg > 1) v = v + ((1 - z) / d * a.bath[nxy + p] - z / d * a.Ey[p]) * w;
+        }
+        U[q] = u; V[q] = v; Wv[q] = w / d;
+    }
+}
+
 // Laplace right-hand side: Dirichlet data P on the free surface of non-ghost columns, zero elsewhere
 // (Runge_Kutta_4.f90:65-69 with no wall flux).
 static __global__ void k_fill_rhs(int nxg, int nyg, int nzg, int gx, int gy, const double *__restrict__ P, double *__restrict__ b) {

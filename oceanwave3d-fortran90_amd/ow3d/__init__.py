@@ -53,7 +53,7 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
            "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
-           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan"]
+           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics"]
 
 
 class OW3DError(RuntimeError):
@@ -98,6 +98,7 @@ def lib():
         L.ow3d_device_bytes.restype = ctypes.c_longlong
         L.ow3d_device_bytes.argtypes = [P]
         L.ow3d_tune.argtypes = [P, I, I]
+        L.ow3d_kinematics.argtypes = [P, D, D, D, D]
         L.ow3d_mg_plan.argtypes = [ctypes.POINTER(Config), IP, IP]
         L.ow3d_step_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), D]
         L.ow3d_estimate_bytes.restype = ctypes.c_longlong
@@ -245,6 +246,14 @@ class Tank:
 
     def replays(self):
         return self.step_stats()[0]
+
+    def kinematics(self, phi=None):
+        """(u, v, w) at every grid point (StoreKinematicData.f90:266-326) from phi (default: the tank's
+        current solution) and the current surface."""
+        p = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape3)
+        U, V, W = (np.empty(self.shape3) for _ in range(3))
+        _check(lib().ow3d_kinematics(self.h, _p(p), _p(U), _p(V), _p(W)), "ow3d_kinematics")
+        return U, V, W
 
     def tune(self, matvec_variant=0, sweep_threads=0):
         """Launch choices (see ow3d_tune in include/ow3d.h); results do not depend on them."""

@@ -60,6 +60,7 @@ def lib():
         L.owo_solve.restype = I
         L.owo_solve.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int), D]
         L.owo_vertical_velocity.argtypes = [P, D, D]
+        L.owo_kinematics.argtypes = [P, D, D, D, D, D]
         L.owo_rhs.argtypes = [P, D, D, D, D, D]
         L.owo_add_relax_zone.argtypes = [P, I, I, I, I, I, D]
         L.owo_rk4_step.restype = I
@@ -198,6 +199,15 @@ class OracleTank:
         W = np.zeros(self.shape2)
         lib().owo_vertical_velocity(self.h, _p(phi), _p(W))
         return W
+
+    def kinematics(self, phi, E):
+        """(u, v, w) at every grid point (StoreKinematicData.f90:266-326) for the surface E last given to
+        surface() / set_state()."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        U, V, W = (np.zeros(self.shape3) for _ in range(3))
+        lib().owo_kinematics(self.h, _p(phi), _p(E), _p(U), _p(V), _p(W))
+        return U, V, W
 
     def rhs(self, E, P, W):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (E, P, W)]

@@ -501,6 +501,36 @@ void owo_vertical_velocity(owo_tank *t, const double *phi, double *W) {
     }
 }
 
+/* Fluid velocities at every grid point (ghost points included) from phi, as StoreKinematicData.f90:266-326
+ * computes them: W = dphi/dsigma (DiffZArbitrary), U = dphi/dx (DiffXEven) and V = dphi/dy (DiffYEven), in
+ * nonlinear mode plus the chain-rule terms ((1 - z)/d h_x - z/d eta_x) W and ((1 - z)/d h_y - z/d eta_y) W
+ * with z the level's sigma coordinate (the ghost level's included, StoreKinematicData.f90:290-297,
+ * 309-316) and d = h + eta; the vertical velocity is W / d (:326).  eta_x, eta_y are those of the last
+ * owo_surface / owo_set_state call, E is that surface. */
+void owo_kinematics(owo_tank *t, const double *phi, const double *E, double *U, double *V, double *Wv) {
+    const grid_t *g = &t->g;
+    const size_t n = g->n;
+    double *W = xcalloc(n, sizeof(double));
+    diff_z(g, phi, W, 1);
+    if (g->nx > 1) diff_x(g, g->nzg, phi, U, 1); else memset(U, 0, n * sizeof(double));
+    if (g->ny > 1) diff_y(g, g->nzg, phi, V, 1); else memset(V, 0, n * sizeof(double));
+    for (int j = 0; j < g->nyg; j++)
+        for (int i = 0; i < g->nxg; i++) {
+            const size_t p = I2(g, i, j);
+            const double d = g->h[p] + E[p];
+            for (int k = 0; k < g->nzg; k++) {
+                const size_t q = I3(g, k, i, j);
+                const double z = g->z[k];
+                if (t->p.nonlinear) {
+                    if (g->nx > 1) U[q] = U[q] + ((1 - z) / d * g->hx[p] - z / d * t->Ex[p]) * W[q];
+                    if (g->ny > 1) V[q] = V[q] + ((1 - z) / d * g->hy[p] - z / d * t->Ey[p]) * W[q];
+                }
+                Wv[q] = W[q] / d;
+            }
+        }
+    free(W);
+}
+
 /* rhsFreeSurface3D.f90:1-29 and rhsLinearFreeSurface3D.f90:1-13 (no applied pressure,
  * no breaking model, no pressure damping: P0 = Qr_x = Mr_t = NuD = Pd = 0). */
 void owo_rhs(owo_tank *t, const double *E, const double *P, const double *W, double *rhsE, double *rhsP) {
