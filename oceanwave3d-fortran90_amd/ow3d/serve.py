@@ -25,9 +25,10 @@ import time
 import numpy as np
 
 import ow3d
-from ow3d import problems
+from ow3d import autotune, problems
 
 PRECISIONS = {"fp64": 0, "mixed": 1}
+_TABLE = autotune.load()
 
 
 class Request:
@@ -42,6 +43,9 @@ class Request:
         self.arrival = float(arrival)
         cfg = problems.batch_cfg(draw)
         cfg.update(mg_precision=self.mg_precision, gmres_m=23, reltol=1e-8, abstol=0.0)
+        self.choice = autotune.tuned(cfg, _TABLE)   # the autotuner's schedule for the class, if cached
+        if self.choice:
+            cfg["mg_maxgrids"] = self.choice["mg_maxgrids"]
         self.cfg = cfg
         self.dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
         self.bytes = ow3d.estimate_bytes(cfg)
@@ -65,6 +69,7 @@ def build_tank(req):
     """The tank of a request in its initial state (the same set-up alone and in the mix)."""
     cfg, bath, E, P, dt, zones = problems.batch_tank_from(req.draw)
     T = ow3d.Tank(req.cfg, **bath)
+    autotune.apply(T, req.choice)
     for z in zones:
         T.add_relax_zone(*z)
     T.set_state(E, P)
