@@ -156,7 +156,7 @@ def main():
             b_mv = 16 * T.n + 8 * 5 * ncol
             from ow3d import autotune
             ch = autotune.tuned(cfg) if MG_LEVELS is None else None
-            mv_kernel = "k_matvec3p" if ch and ch["matvec_variant"] == 2 else "k_matvec3c"
+            mv_kernel = {2: "k_matvec3p", 3: "k_matvec3s"}.get(ch["matvec_variant"] if ch else 0, "k_matvec3c")
         T.close()
     dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
     mix, f64 = results["mixed"], results["fp64"]

@@ -128,7 +128,8 @@ int ow3d_wait(ow3d_tank *t);
 int ow3d_step_stats(const ow3d_tank *t, long long *replays, long long *captures, double *capture_ms);
 /* Launch choices of a handle, set by the autotuner (ow3d/autotune.py; no counterpart in the reference):
  * matvec_variant 0 column kernel with the widest tile fitting 64 KiB of LDS, 1 column kernel with 16 x 8
- * tiles, 2 point kernel; sweep_threads 0 automatic (1024 below 256 tiles, else 256), 256 or 1024.
+ * tiles, 2 point kernel, 3 column kernel with each column split over two waves; sweep_threads 0
+ * automatic (1024 below 256 tiles, else 256), 256 or 1024.
  * Results do not depend on them. */
 int ow3d_tune(ow3d_tank *t, int matvec_variant, int sweep_threads);
 /* Device memory held by a handle, and what ow3d_create would allocate for cfg (admission control). */

@@ -755,7 +755,7 @@ extern "C" int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, doubl
 
 extern "C" int ow3d_tune(ow3d_tank *t, int matvec_variant, int sweep_threads) {
     if (!t) { err() = "null handle"; return -1; }
-    if (matvec_variant < 0 || matvec_variant > 2) { err() = "matvec variant must be 0, 1 or 2"; return -1; }
+    if (matvec_variant < 0 || matvec_variant > 3) { err() = "matvec variant must be 0, 1, 2 or 3"; return -1; }
     if (sweep_threads != 0 && sweep_threads != 256 && sweep_threads != 1024) { err() = "sweep threads must be 0, 256 or 1024"; return -1; }
     if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
     HIPCHK(hipStreamSynchronize(t->s));

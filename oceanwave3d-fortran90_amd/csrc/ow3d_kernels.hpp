@@ -419,6 +419,206 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
     }
 }
 
+// The column matvec with the work of each column split between two waves: a block of 16 x 16 columns
+// has 512 threads; threads 0..255 (waves 0-3) take the x-side of column t (d/dx, d2/dx2, the sigma
+// derivatives of phi and the mixed x-sigma term, the bottom row), threads 256..511 (waves 4-7) the
+// y-side of column t - 256 (d/dy, d2/dy2, the mixed y-sigma term).  The y-side leaves its part of every
+// row in LDS; the x-side adds it.  Each thread holds half the live values of the one-thread-per-column
+// kernel and a block has twice the waves.  Same terms as k_matvec3c; the x and y parts are summed in
+// another order (results agree to rounding).
+template <int NZ, int A>
+__global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+    if (solve_done(g.skip)) return;
+    constexpr int TX = 16, TY = 16, NC = TX * TY, NT = 2 * NC, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
+    constexpr int PS = (WX * WY) | 1;
+    __shared__ double sP[NZ * PS];
+    __shared__ double sSX[(TX + 2) * 2 * RA];
+    __shared__ double sSY[(TY + 2) * 2 * RA];
+    __shared__ double sYp[NZ * NC];   // y-side part of each row: [k][column]
+    const int nxg = g.nxg, nyg = g.nyg, nxy = g.nxy;
+    const int gxt = (nxg + TX - 1) / TX;
+    const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int i0 = (tile % gxt) * TX, j0 = (tile / gxt) * TY;
+    const int lox = cmax(cmin(i0 - R, nxg - 2 - 2 * A), 0);
+    const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
+    const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
+    {
+        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT + NT - 1) / NT;
+        double v[PER];
+        const double *src = phi + ((size_t)loy * nxg + lox) * NZ;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
+            v[q] = (e < TOT && r < wy && c < wx * NZ) ? src[(size_t)r * nxg * NZ + c] : 0.0;
+        }
+        for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
+            const int r = e / (2 * RA), ii = i0 - 1 + r;
+            if (ii >= 0 && ii < nxg) sSX[e] = g.sx[(size_t)ii * 2 * RA + (e - r * 2 * RA)];
+        }
+        for (int e = threadIdx.x; e < (TY + 2) * 2 * RA; e += NT) {
+            const int r = e / (2 * RA), jj = j0 - 1 + r;
+            if (jj >= 0 && jj < nyg) sSY[e] = g.sy[(size_t)jj * 2 * RA + (e - r * 2 * RA)];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
+            if (e < TOT && r < wy && c < wx * NZ) {
+                const int cc = c / NZ, k = c - cc * NZ;
+                sP[k * PS + r * WX + cc] = v[q];
+            }
+        }
+    }
+    __syncthreads();
+    auto P = [&](int k, int ii, int jj) -> double { return sP[k * PS + (jj - loy) * WX + (ii - lox)]; };
+    auto SX = [&](int ii) -> const double * { return sSX + (ii - (i0 - 1)) * 2 * RA; };
+    auto SY = [&](int jj) -> const double * { return sSY + (jj - (j0 - 1)) * 2 * RA; };
+    const bool yside = threadIdx.x >= NC;
+    const int col = yside ? (int)threadIdx.x - NC : (int)threadIdx.x;
+    const int tx = col % TX, ty = col / TX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = (i < nxg) && (j < nyg);
+    const bool wxl = (i == 0 || i == nxg - 1), wyl = (j == 0 || j == nyg - 1);
+    const bool interior = active && !wxl && !wyl;
+    double o[NZ];
+    if (yside) {
+        if (interior) {
+            const int wsy = cclamp(j - A, 0, nyg - RA);
+            double dy[NZ];
+            const double *S1 = SY(j), *S2 = S1 + RA;
+#pragma unroll
+            for (int k = 0; k < NZ; k++) {
+                double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) { const double v = P(k, i, wsy + m); d1 += S1[m] * v; d2 += S2[m] * v; }
+                dy[k] = d1; o[k] = d2;
+            }
+            double dyc0 = dy[0];
+            {
+                int sh = 0;
+                if (wsy == 0) sh = 1;
+                if (wsy + RA - 1 == nyg - 1) sh = -1;
+                if (sh != 0) {
+                    const double *S = SY(j - sh);
+                    double t = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) t += S[m] * P(0, i, wsy + sh + m);
+                    dyc0 = t;
+                }
+            }
+            const double *cfp = g.cf + i + (size_t)nxg * j;
+            const double A4 = cfp[2 * (size_t)nxy], B4 = cfp[3 * (size_t)nxy], S5 = cfp[6 * (size_t)nxy];
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const int wz = cclamp(k - A, 0, NZ - RA);
+                const double *Z1 = g.sz + (k * 2) * RA;
+                double dyz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) dyz += Z1[m] * ((wz + m == 0) ? dyc0 : dy[wz + m]);
+                const double s4 = A4 + g.zs[k] * B4;
+                o[k] = o[k] + 2.0 * (s4 * dyz);
+            }
+            const double d = 1.0 / S5;
+            o[0] = (A4 * d) * dy[1];   // h_y dy at the bottom row (h_y = A4 d, d = 1 / S5)
+#pragma unroll
+            for (int k = 0; k < NZ - 1; k++) sYp[k * NC + col] = o[k];
+        }
+    } else if (active) {
+        if (wxl && wyl) {
+#pragma unroll
+            for (int k = 0; k < NZ; k++) o[k] = P(k, i, j);
+        } else if (wxl || wyl) {
+            const int ib = wxl ? ((i == 0) ? 1 : nxg - 2) : ((j == 0) ? 1 : nyg - 2);
+            const int ws = wxl ? cclamp(ib - A, 0, nxg - RA) : cclamp(ib - A, 0, nyg - RA);
+            const double *S = wxl ? SX(ib) : SY(ib);
+            o[0] = P(0, i, j);
+#pragma unroll
+            for (int k = 1; k < NZ; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) s += S[m] * (wxl ? P(k, ws + m, j) : P(k, i, ws + m));
+                o[k] = s;
+            }
+        } else {
+            double dx[NZ], pc[NZ];
+            const int wsx = cclamp(i - A, 0, nxg - RA);
+            {
+                const double *S1 = SX(i), *S2 = S1 + RA;
+#pragma unroll
+                for (int k = 0; k < NZ; k++) {
+                    double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) { const double v = P(k, wsx + m, j); d1 += S1[m] * v; d2 += S2[m] * v; }
+                    dx[k] = d1; o[k] = d2;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NZ; k++) pc[k] = P(k, i, j);
+            double dxc0 = dx[0];
+            {
+                int sh = 0;
+                if (wsx == 0) sh = 1;
+                if (wsx + RA - 1 == nxg - 1) sh = -1;
+                if (sh != 0) {
+                    const double *S = SX(i - sh);
+                    double t = 0.0;
+#pragma unroll
+                    for (int m = 0; m < RA; m++) t += S[m] * P(0, wsx + sh + m, j);
+                    dxc0 = t;
+                }
+            }
+            const double *cfp = g.cf + i + (size_t)nxg * j;
+            const double A2 = cfp[0], B2 = cfp[nxy], A4 = cfp[2 * (size_t)nxy], B4 = cfp[3 * (size_t)nxy];
+            const double A3 = cfp[4 * (size_t)nxy], B3 = cfp[5 * (size_t)nxy], S5 = cfp[6 * (size_t)nxy];
+            {
+                constexpr int wz = (1 - A) < 0 ? 0 : (1 - A);
+                const double *Z1 = g.sz + (1 * 2) * RA;
+                double dz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) dz += Z1[m] * pc[wz + m];
+                const double sg = g.zs[1], d = 1.0 / S5, hx = A2 * d, hy = A4 * d;
+                o[0] = (S5 + hx * (A2 + sg * B2) + hy * (A4 + sg * B4)) * dz + hx * dx[1];
+            }
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const int wz = cclamp(k - A, 0, NZ - RA);
+                const double *Z1 = g.sz + (k * 2) * RA, *Z2 = Z1 + RA;
+                double dz = 0.0, dzz = 0.0, dxz = 0.0;
+#pragma unroll
+                for (int m = 0; m < RA; m++) {
+                    const double z1 = Z1[m], cv = pc[wz + m];
+                    dz += z1 * cv;
+                    dzz += Z2[m] * cv;
+                    dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
+                }
+                const double sg = g.zs[k];
+                const double s2 = A2 + sg * B2, s4 = A4 + sg * B4, s3 = A3 + sg * B3;
+                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz) + (s2 * s2 + s4 * s4 + S5 * S5) * dzz;
+            }
+            o[NZ - 1] = pc[NZ - 1];
+        }
+    }
+    __syncthreads();   // every read of the phi window and every y-side part done
+    if (!yside && active) {
+        if (interior) {
+#pragma unroll
+            for (int k = 0; k < NZ - 1; k++) o[k] += sYp[k * NC + col];
+        }
+#pragma unroll
+        for (int k = 0; k < NZ; k++) sP[k * PS + (j - loy) * WX + (i - lox)] = o[k];
+    }
+    __syncthreads();
+    {
+        const int ie = cmin(i0 + TX, nxg), je = cmin(j0 + TY, nyg);
+        for (int jj = j0; jj < je; jj++) {
+            double *dst = out + ((size_t)jj * nxg + i0) * NZ;
+            for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
+                const int ii = e / NZ, k = e - ii * NZ;
+                dst[e] = sP[k * PS + (jj - loy) * WX + (i0 + ii - lox)];
+            }
+        }
+    }
+}
+
 // x-z problems (Ny == 1), one thread per grid point: block = 64 columns x NZ levels.
 template <int NZ, int A>
 __global__ __launch_bounds__(256) void k_matvec2p(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {

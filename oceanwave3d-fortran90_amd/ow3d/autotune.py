@@ -6,8 +6,8 @@ tank under each candidate and keeps the fastest:
 * the V-cycle schedule: the number of multigrid levels (MGmaxgrids), from the full hierarchy down to the
   fewest levels whose coarsest level the dense solve accepts (at most 4096 unknowns, ow3d.mg_plan);
   fewer levels trade launch-latency-bound small levels for a larger dense coarse solve;
-* the matvec tile (ow3d_tune: column kernel with the widest tile, with 16 x 8 tiles, or the point kernel),
-  timed with ow3d_time_matvec;
+* the matvec kernel (ow3d_tune: column kernel with the widest tile, with 16 x 8 tiles, with each column
+  split over two waves, or the point kernel), timed with ow3d_time_matvec;
 * the block size of the zebra sweeps (automatic, 256 or 1024 threads).
 
 None of these changes results beyond the multigrid depth, which is the reference's own MGmaxgrids
@@ -99,7 +99,7 @@ def tune_class(spec, mg_precision=1, warmup=3, steps=3, log=None):
             T.close()
     T, _, L = best
     mv = {}
-    for v in ((0, 1, 2) if base["ny"] > 1 else (0,)):
+    for v in ((0, 1, 2, 3) if base["ny"] > 1 else (0,)):
         T.tune(v, 0)
         T.time_matvec(3)
         mv[v] = 1e3 * T.time_matvec(30)

@@ -69,6 +69,22 @@ def test_matvec_parity(oracle_mod, nx, ny, nz, shoal, order):
     assert rel(yg, yo) < 1e-12, (rel(yg, yo), np.unravel_index(np.argmax(np.abs(yg - yo)), yo.shape))
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("nx,ny,nz,shoal,order", [c for c in CASES if c[1] > 1])
+def test_matvec_variants_parity(oracle_mod, nx, ny, nz, shoal, order, variant):
+    """The autotuner's other matvec kernels (16 x 8 column tiles, point kernel, columns split over two
+    waves) against the oracle, on every 3D case."""
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, order=order, precond=0)
+    T_g.tune(variant, 0)
+    E, P = surface(nx, ny, 10.0)
+    T_o.surface(E, P)
+    T_g.set_surface(E, P)
+    phi = np.random.default_rng(1).standard_normal(T_o.shape3)
+    yo = T_o.matvec(phi)
+    yg = T_g.laplace_apply(phi)
+    assert rel(yg, yo) < 1e-12, (rel(yg, yo), np.unravel_index(np.argmax(np.abs(yg - yo)), yo.shape))
+
+
 @pytest.mark.parametrize("nx,ny,nz,shoal", [(33, 33, 9, True), (65, 1, 9, True), (17, 17, 5, False)])
 def test_vertical_velocity_and_rhs(oracle_mod, nx, ny, nz, shoal):
     T_o, T_g, _ = make_pair(oracle_mod, nx, ny, nz, shoal=shoal, order=4 if nz < 7 else 6, precond=0)
