@@ -191,14 +191,14 @@ def test_request_loop_respects_the_memory_budget():
 
 
 def test_launch_choices_do_not_change_results():
-    """Zebra-sweep block size only changes the thread mapping (bit-identical); the point matvec kernel
-    sums the same terms in another order (within 1e-11)."""
+    """Zebra-sweep block size only changes the thread mapping (bit-identical); the other matvec kernels
+    sum the same terms in another order (within 1e-11)."""
     from ow3d import problems
     spec = problems.batch_tank(71, sizes=(65,), nzs=(9,), orders=(6,), bottom="shoal")
     ref, dt = make(spec)
     ref.rk4_steps(4, 0.0, dt)
     Eref = ref.get_state()
-    for mv, thr, exact in ((0, 256, True), (0, 1024, True), (1, 0, False), (2, 0, False)):
+    for mv, thr, exact in ((0, 256, True), (0, 1024, True), (1, 0, False), (2, 0, False), (3, 0, False)):
         T, _ = make(spec)
         T.tune(mv, thr)
         T.rk4_steps(4, 0.0, dt)
@@ -210,4 +210,4 @@ def test_launch_choices_do_not_change_results():
                 assert np.max(np.abs(a - b)) <= 1e-11 * max(np.max(np.abs(b)), 1e-300), (mv, thr)
         T.close()
     with pytest.raises(Exception):
-        ref.tune(3, 0)
+        ref.tune(4, 0)
