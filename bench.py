@@ -102,6 +102,10 @@ def main():
                     help="multigrid levels of the 513^2 tank (MGmaxgrids); default: the autotuner's cached choice, else 7")
     ap.add_argument("--batched-tanks", type=int, default=2,
                     help="also advance this many 513x513x9 tanks together (different JONSWAP seeds); 0: skip")
+    ap.add_argument("--sweep-threads", type=int, default=None,
+                    help="with --profile-precond: zebra sweep launch choice instead of the autotuner's (see ow3d_tune)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the --batched-tanks run of the 513x513x9 tanks (prints its own JSON line)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="many-tank workload instead: N uneven tanks advanced together (prints its own JSON line)")
     ap.add_argument("--untuned", action="store_true", help="--batch without the autotuner's cached schedules")
@@ -129,8 +133,15 @@ def main():
     if args.serve:
         print(json.dumps(bench_serve(args.serve)), flush=True)
         return
+    if args.batched_only:
+        print(json.dumps(bench_batched_513(ow3d, problems, args.batched_tanks, args.steps, args.warmup)), flush=True)
+        return
     if args.profile_precond:
         T, cfg, dt = make_tank(ow3d, problems, 1)
+        if args.sweep_threads is not None:
+            from ow3d import autotune
+            ch = autotune.tuned(cfg)
+            T.tune(ch["matvec_variant"] if ch else 0, args.sweep_threads)
         r = np.random.default_rng(0).standard_normal(T.shape3)
         for _ in range(args.profile_precond):
             T.precond_apply(r)
@@ -211,7 +222,9 @@ def main():
 
 def bench_batched_513(ow3d, problems, k, steps, warmup):
     """k tanks of the 513x513x9 workload (JONSWAP seeds 0 .. k-1) advanced together with
-    ow3d.batch_rk4_steps, FP64 and FP32 multigrid: DOF-steps/s of the whole batch and its step latency."""
+    ow3d.batch_rk4_steps, FP64 and FP32 multigrid: DOF-steps/s of the whole batch over `steps` steps in one
+    call (each tank starts its next step as soon as its previous one has settled), then the latency of a
+    step of the whole batch (one call per step, so every step waits for the slowest tank)."""
     out = {"tanks": k}
     for label, prec in (("fp64", 0), ("mixed", 1)):
         tanks, dts = [], []
@@ -219,13 +232,17 @@ def bench_batched_513(ow3d, problems, k, steps, warmup):
             T, cfg, dt = make_tank(ow3d, problems, prec, seed=s)
             tanks.append(T); dts.append(dt)
         ow3d.batch_rk4_steps(tanks, warmup, dts)
+        a = time.perf_counter()
+        ow3d.batch_rk4_steps(tanks, steps, dts)
+        t_all = time.perf_counter() - a
         lat = []
         for _ in range(steps):
             a = time.perf_counter()
             ow3d.batch_rk4_steps(tanks, 1, dts)
             lat.append(time.perf_counter() - a)
         dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1) * k
-        out[label] = {"dof_steps_per_s": dof * steps / sum(lat), "ms_per_batch_step": 1e3 * sum(lat) / steps,
+        out[label] = {"dof_steps_per_s": dof * steps / t_all, "ms_per_batch_step": 1e3 * t_all / steps,
+                      "dof_steps_per_s_stepwise": dof * steps / sum(lat),
                       "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99))}
         for T in tanks:
             T.close()

@@ -52,25 +52,38 @@ __device__ __forceinline__ void column_coefs(double d, double Ex, double Ey, dou
 // Copies a window of rows x cols sigma-columns (NZ values each) of a field in HBM into LDS: window row r
 // starts at src + r * pitch; LDS holds element (r, c, k) at (r * RS + c) * NZP + k.  All loads of a thread
 // are issued before its first LDS store, so a block waits for HBM once, not once per row.
-template <typename T, int NZ, int NZP, int MR, int MC, int RS, int NT>
-__device__ __forceinline__ void stage_window(T *lds, const T *__restrict__ src, size_t pitch, int rows, int cols) {
-    constexpr int RL = MC * NZ, TOT = MR * RL, PER = (TOT + NT - 1) / NT;
+// The two halves are also available separately (WindowRegs: load into registers, store to LDS) so that a
+// kernel can issue the loads of several windows before it waits for any of them.
+template <typename T, int NZ, int MR, int MC, int NT>
+struct WindowRegs {
+    static constexpr int RL = MC * NZ, TOT = MR * RL, PER = (TOT + NT - 1) / NT;
     T v[PER];
+    __device__ __forceinline__ void load(const T *__restrict__ src, size_t pitch, int rows, int cols) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int r = e / RL, c = e - r * RL;
-        v[q] = (e < TOT && r < rows && c < cols * NZ) ? src[(size_t)r * pitch + c] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int r = e / RL, c = e - r * RL;
-        if (e < TOT && r < rows && c < cols * NZ) {
-            const int cc = c / NZ, k = c - cc * NZ;
-            lds[(r * RS + cc) * NZP + k] = v[q];
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int r = e / RL, c = e - r * RL;
+            v[q] = (e < TOT && r < rows && c < cols * NZ) ? src[(size_t)r * pitch + c] : T(0);
         }
     }
+    template <int NZP, int RS>
+    __device__ __forceinline__ void store(T *lds, int rows, int cols) const {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int r = e / RL, c = e - r * RL;
+            if (e < TOT && r < rows && c < cols * NZ) {
+                const int cc = c / NZ, k = c - cc * NZ;
+                lds[(r * RS + cc) * NZP + k] = v[q];
+            }
+        }
+    }
+};
+template <typename T, int NZ, int NZP, int MR, int MC, int RS, int NT>
+__device__ __forceinline__ void stage_window(T *lds, const T *__restrict__ src, size_t pitch, int rows, int cols) {
+    WindowRegs<T, NZ, MR, MC, NT> w;
+    w.load(src, pitch, rows, cols);
+    w.template store<NZP, RS>(lds, rows, cols);
 }
 
 // Block index -> tile index such that each of the 8 XCDs (which receive blocks round-robin) works on a
@@ -1282,6 +1295,8 @@ struct LvArgs {
     const T *zs;            // sigma of each level (0 on the ghost level)
     const T *col;           // 6 planes: 1/h, h_x, h_y, h_x/h, h_y/h, -2(h_x^2+h_y^2)/h^2 + (h_xx+h_yy)/h
     const int *skip;        // solve_done flag (nullptr: always run)
+    int spec;               // 1: the kernels issue their loads before testing skip (small levels, where a
+                            // launch is a chain of memory latencies); 0: skip is tested first
 };
 
 // Level coefficients as the column kernels read them: 3-point stencil rows sx(i), sy(j), sz(k) (first then
@@ -1317,47 +1332,61 @@ struct CoefLds {
 // Stages the coefficients of the window [lox, lox+wcols) x [loy, loy+wrows) (row length MW) into buf
 // (CoefLds<MW, MH, NZ>::TOTAL elements) and returns the accessor.  Each segment is one predicated load per
 // element with no branches around it, so all loads of a thread are in flight together.
+// CoefRegs holds the two halves separately (load into registers, store to LDS), as WindowRegs.
+template <typename T, int NZ, int MW, int MH, int NT>
+struct CoefRegs {
+    using S = CoefLds<MW, MH, NZ>;
+    static constexpr int PX = (S::SX + NT - 1) / NT, PY = (S::SY + NT - 1) / NT, PZ = (S::SZ + S::ZS + NT - 1) / NT;
+    static constexpr int PC = (S::COL + NT - 1) / NT;
+    T vx[PX], vy[PY], vz[PZ], vc[PC];
+    int lox, loy;
+    __device__ __forceinline__ void load(const LvArgs<T> &L, int lox_, int loy_, int wcols, int wrows) {
+        lox = lox_; loy = loy_;
+        const int t = (int)threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < PX; q++) {
+            const int e = t + q * NT;
+            vx[q] = (e < S::SX && e / 6 < wcols) ? L.sx[lox * 6 + e] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PY; q++) {
+            const int e = t + q * NT;
+            vy[q] = (e < S::SY && e / 6 < wrows) ? L.sy[loy * 6 + e] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PZ; q++) {
+            const int e = t + q * NT;
+            vz[q] = e < S::SZ ? L.sz[e] : (e < S::SZ + S::ZS ? L.zs[e - S::SZ] : T(0));
+        }
+#pragma unroll
+        for (int q = 0; q < PC; q++) {
+            const int e = t + q * NT;
+            const int pl = e / (MW * MH), c = e - pl * (MW * MH), r = c / MW, cc = c - r * MW;
+            vc[q] = (e < S::COL && r < wrows && cc < wcols) ? L.col[(size_t)pl * L.nxy + (size_t)(loy + r) * L.nxg + lox + cc] : T(0);
+        }
+    }
+    __device__ __forceinline__ CoefS<T> store(T *buf) const {
+        const int t = (int)threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < PX; q++) { const int e = t + q * NT; if (e < S::SX) buf[e] = vx[q]; }
+#pragma unroll
+        for (int q = 0; q < PY; q++) { const int e = t + q * NT; if (e < S::SY) buf[S::SX + e] = vy[q]; }
+#pragma unroll
+        for (int q = 0; q < PZ; q++) { const int e = t + q * NT; if (e < S::SZ + S::ZS) buf[S::SX + S::SY + e] = vz[q]; }
+#pragma unroll
+        for (int q = 0; q < PC; q++) { const int e = t + q * NT; if (e < S::COL) buf[S::SX + S::SY + S::SZ + S::ZS + e] = vc[q]; }
+        CoefS<T> C;
+        C.sx_ = buf; C.sy_ = buf + S::SX; C.sz_ = buf + S::SX + S::SY; C.zs_ = buf + S::SX + S::SY + S::SZ;
+        C.col_ = buf + S::SX + S::SY + S::SZ + S::ZS;
+        C.ib = lox; C.jb = loy; C.nx = MW; C.np = MW * MH;
+        return C;
+    }
+};
 template <typename T, int NZ, int MW, int MH, int NT>
 __device__ __forceinline__ CoefS<T> stage_coefs(const LvArgs<T> &L, int lox, int loy, int wcols, int wrows, T *buf) {
-    using S = CoefLds<MW, MH, NZ>;
-    constexpr int PX = (S::SX + NT - 1) / NT, PY = (S::SY + NT - 1) / NT, PZ = (S::SZ + S::ZS + NT - 1) / NT;
-    constexpr int PC = (S::COL + NT - 1) / NT;
-    const int t = (int)threadIdx.x;
-    T vx[PX], vy[PY], vz[PZ], vc[PC];
-#pragma unroll
-    for (int q = 0; q < PX; q++) {
-        const int e = t + q * NT;
-        vx[q] = (e < S::SX && e / 6 < wcols) ? L.sx[lox * 6 + e] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < PY; q++) {
-        const int e = t + q * NT;
-        vy[q] = (e < S::SY && e / 6 < wrows) ? L.sy[loy * 6 + e] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < PZ; q++) {
-        const int e = t + q * NT;
-        vz[q] = e < S::SZ ? L.sz[e] : (e < S::SZ + S::ZS ? L.zs[e - S::SZ] : T(0));
-    }
-#pragma unroll
-    for (int q = 0; q < PC; q++) {
-        const int e = t + q * NT;
-        const int pl = e / (MW * MH), c = e - pl * (MW * MH), r = c / MW, cc = c - r * MW;
-        vc[q] = (e < S::COL && r < wrows && cc < wcols) ? L.col[(size_t)pl * L.nxy + (size_t)(loy + r) * L.nxg + lox + cc] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < PX; q++) { const int e = t + q * NT; if (e < S::SX) buf[e] = vx[q]; }
-#pragma unroll
-    for (int q = 0; q < PY; q++) { const int e = t + q * NT; if (e < S::SY) buf[S::SX + e] = vy[q]; }
-#pragma unroll
-    for (int q = 0; q < PZ; q++) { const int e = t + q * NT; if (e < S::SZ + S::ZS) buf[S::SX + S::SY + e] = vz[q]; }
-#pragma unroll
-    for (int q = 0; q < PC; q++) { const int e = t + q * NT; if (e < S::COL) buf[S::SX + S::SY + S::SZ + S::ZS + e] = vc[q]; }
-    CoefS<T> C;
-    C.sx_ = buf; C.sy_ = buf + S::SX; C.sz_ = buf + S::SX + S::SY; C.zs_ = buf + S::SX + S::SY + S::SZ;
-    C.col_ = buf + S::SX + S::SY + S::SZ + S::ZS;
-    C.ib = lox; C.jb = loy; C.nx = MW; C.np = MW * MH;
-    return C;
+    CoefRegs<T, NZ, MW, MH, NT> c;
+    c.load(L, lox, loy, wcols, wrows);
+    return c.store(buf);
 }
 
 // Coefficients of the linearised (eta = 0) level operator at sigma level k: with eta = 0 the sigma
@@ -1924,7 +1953,7 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // (ghost pass, red, black, ghost pass) with x read once and written once.
 template <typename T, int NZ, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
-    if (solve_done(L.skip)) return;
+    if (!L.spec && solve_done(L.skip)) return;
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
     __shared__ T sX[WX * WY * NZP];
@@ -1943,20 +1972,29 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     const int je = d3 ? L.gy + cmin(nyi, ((int)blockIdx.y + 1) * TY - offy) : 1;
     const int lox = cmax(i0 - H, 0), loy = d3 ? cmax(j0 - H, 0) : 0;
     const int wcols = cmin(ie + H, nxg) - lox, wrows = d3 ? cmin(je + H, nyg) - loy : 1;
-    const CoefS<T> C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
-    stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
     // b over the ring-extended tile [i0-1, ie] x [j0-1, je] (clipped to the grid)
     const int blox = i0 - 1, bloy = d3 ? j0 - 1 : 0;
-    stage_window<T, NZ, NZ, TY + 2, TX + 2, TX + 2, NT>(sB, b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ,
-                                                         d3 ? cmin(je + 1, nyg) - bloy : 1, cmin(ie + 1, nxg) - blox);
+    const int bcols = cmin(ie + 1, nxg) - blox, brows = d3 ? cmin(je + 1, nyg) - bloy : 1;
+    // every HBM load of the block (coefficients, x window, b ring) is in flight before the first wait
+    CoefRegs<T, NZ, WX, WY, NT> cr;
+    WindowRegs<T, NZ, WY, WX, NT> xr;
+    WindowRegs<T, NZ, TY + 2, TX + 2, NT> br;
+    cr.load(L, lox, loy, wcols, wrows);
+    xr.load(xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    br.load(b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
+    if (L.spec && solve_done(L.skip)) return;
+    const CoefS<T> C = cr.store(sC);
+    xr.template store<NZP, WX>(sX, wrows, wcols);
+    br.template store<NZ, TX + 2>(sB, brows, bcols);
     __syncthreads();
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
     const int jlo = d3 ? L.gy : 0, jhi = d3 ? nyg - L.gy : 1;
-    // value of wall ghost point (ii, jj, k) from the staged interior values (GaussSeidelSpecialPoints.f90 rows:
-    // identity on the sigma ghost level and at corners, the Neumann row relaxed pointwise elsewhere)
+    // value of wall ghost point (ii, jj, k) of the ring-extended tile from the staged interior values
+    // (GaussSeidelSpecialPoints.f90 rows: identity on the sigma ghost level and at corners, the Neumann row
+    // relaxed pointwise elsewhere)
     auto ghost_value = [&](int ii, int jj, int k) -> T {
         const bool wxg = L.gx && (ii == 0 || ii == nxg - 1), wyg = L.gy && (jj == 0 || jj == nyg - 1);
-        const T bq = b[((size_t)jj * nxg + ii) * NZ + k];
+        const T bq = sB[((jj - bloy) * (TX + 2) + (ii - blox)) * NZ + k];
         if ((wxg && wyg) || k == 0) return bq;
         T sum = T(0), dg = T(1);
         if (wxg) {
@@ -1981,11 +2019,13 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     auto is_ghost = [&](int ii, int jj) -> bool {
         return (L.gx && (ii == 0 || ii == nxg - 1)) || (L.gy && (jj == 0 || jj == nyg - 1));
     };
-    // the ghost pass before the sweep, on the window (ghost values depend on interior values only)
-    if (lox == 0 || lox + wcols == nxg || (d3 && (loy == 0 || loy + wrows == nyg))) {
-        for (int e = threadIdx.x; e < wrows * wcols * NZ; e += NT) {
-            const int r = e / (wcols * NZ), rem = e - r * wcols * NZ, c = rem / NZ, k = rem - c * NZ;
-            const int ii = lox + c, jj = loy + r;
+    // the ghost pass before the sweep, on the ring-extended tile (ghost values depend on interior values
+    // only; the sweep reads no ghost point outside that ring: a relaxed ring column's outer neighbour is
+    // interior, since no tile is one column or row wide)
+    if (blox == 0 || blox + bcols == nxg || (d3 && (bloy == 0 || bloy + brows == nyg))) {
+        for (int e = threadIdx.x; e < brows * bcols * NZ; e += NT) {
+            const int r = e / (bcols * NZ), rem = e - r * bcols * NZ, c = rem / NZ, k = rem - c * NZ;
+            const int ii = blox + c, jj = bloy + r;
             if (is_ghost(ii, jj)) {
                 const T v = ghost_value(ii, jj, k);
                 sX[((jj - loy) * WX + (ii - lox)) * NZP + k] = v;
@@ -2075,7 +2115,7 @@ __global__ void k_mg_ghost_p(LvArgs<T> L, T *__restrict__ x, const T *__restrict
 // problems: one row of 16 * TY columns), level coefficients staged in LDS with the tile.
 template <typename T, int NZ, int TY>
 __global__ __launch_bounds__(16 * TY) void k_mg_residual_t(LvArgs<T> L, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r) {
-    if (solve_done(L.skip)) return;
+    if (!L.spec && solve_done(L.skip)) return;
     constexpr int TX = 16, R = 2, WX = TX + 2 * R, WY = TY + 2 * R, NZP = NZ | 1, NT = TX * TY;
     constexpr int RS2 = TX * TY + 2 * R;       // x-z problems: the tile is one row of TX * TY columns
     constexpr int LX = WX * WY > RS2 ? WX * WY : RS2;
@@ -2088,21 +2128,37 @@ __global__ __launch_bounds__(16 * TY) void k_mg_residual_t(LvArgs<T> L, const T 
     const int loy = d3 ? cmax(cmin(j0 - R, nyg - 4), 0) : 0;
     const int wcols = cmin(tw + 2 * R, nxg - lox), wrows = d3 ? cmin(WY, nyg - loy) : 1;
     const int rowstride = d3 ? WX : RS2;
-    CoefS<T> C;
-    if (d3) {
-        C = stage_coefs<T, NZ, WX, WY, NT>(L, lox, loy, wcols, wrows, sC);
-        stage_window<T, NZ, NZP, WY, WX, WX, NT>(sX, x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
-    } else {
-        C = stage_coefs<T, NZ, RS2, 1, NT>(L, lox, loy, wcols, wrows, sC);
-        stage_window<T, NZ, NZP, 1, RS2, RS2, NT>(sX, x + (size_t)lox * NZ, 0, 1, wcols);
-    }
-    __syncthreads();
-    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * rowstride + (ii - lox)) * NZP + k]; };
     const int tx = d3 ? (int)threadIdx.x % TX : (int)threadIdx.x, ty = d3 ? (int)threadIdx.x / TX : 0;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = i < nxg && j < nyg;
+    // every HBM load of the block (coefficients, x window, the thread's column of b) is in flight before
+    // the first wait
+    T bc[NZ];
+    const T *bp = b + ((size_t)i + (size_t)nxg * j) * NZ;
+#pragma unroll
+    for (int k = 0; k < NZ; k++) bc[k] = active ? bp[k] : T(0);
+    CoefS<T> C;
+    if (d3) {
+        CoefRegs<T, NZ, WX, WY, NT> cr;
+        WindowRegs<T, NZ, WY, WX, NT> xr;
+        cr.load(L, lox, loy, wcols, wrows);
+        xr.load(x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+        if (L.spec && solve_done(L.skip)) return;
+        C = cr.store(sC);
+        xr.template store<NZP, WX>(sX, wrows, wcols);
+    } else {
+        CoefRegs<T, NZ, RS2, 1, NT> cr;
+        WindowRegs<T, NZ, 1, RS2, NT> xr;
+        cr.load(L, lox, loy, wcols, wrows);
+        xr.load(x + (size_t)lox * NZ, 0, 1, wcols);
+        if (L.spec && solve_done(L.skip)) return;
+        C = cr.store(sC);
+        xr.template store<NZP, RS2>(sX, 1, wcols);
+    }
+    __syncthreads();
+    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * rowstride + (ii - lox)) * NZP + k]; };
     T o[NZ];
-    if (active) residual_column<T, NZ>(L, C, i, j, X, b + ((size_t)i + (size_t)nxg * j) * NZ, o);
+    if (active) residual_column<T, NZ>(L, C, i, j, X, bc, o);
     __syncthreads();
     if (active) {
         T *dst = sX + ((j - loy) * rowstride + (i - lox)) * NZP;
@@ -2128,18 +2184,16 @@ struct RestrictGeo {
     int xc, yc, zc;
     const double *xf, *yf, *zf;                      // fine coordinates
     const int *skip;                                 // solve_done flag (nullptr: always run)
+    int spec;                                        // 1: loads are issued before skip is tested
 };
-// Coarse point q; also clears the coarse iterate xz (the coarse V cycle starts from zero).
+// Restricted residual at coarse point q < nxgc * nygc * nzgc (0 on ghost points).
 template <typename T>
-__device__ __forceinline__ void restrict_point(const RestrictGeo &G, size_t q, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
-    const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
-    if (q >= nc) return;
-    xz[q] = T(0);
+__device__ __forceinline__ T restrict_value(const RestrictGeo &G, size_t q, const T *__restrict__ rf) {
     const int kc = (int)(q % G.nzgc);
     const size_t col = q / G.nzgc;
     const int ic = (int)(col % G.nxgc), jc = (int)(col / G.nxgc);
     const int icn = ic - G.gx, jcn = jc - G.gy;
-    if (kc < 1 || icn < 0 || icn >= G.nxc || jcn < 0 || jcn >= G.nyc) { rc[q] = T(0); return; }
+    if (kc < 1 || icn < 0 || icn >= G.nxc || jcn < 0 || jcn >= G.nyc) return T(0);
     const int kf = 1 + (kc - 1) * (1 + G.zc), ifn = G.gx + icn * (1 + G.xc), jfn = G.gy + jcn * (1 + G.yc);
     auto R = [&](int k, int i, int j) -> T { return rf[((size_t)j * G.nxgf + i) * G.nzgf + k]; };
     // x pass value at (k,i,j)
@@ -2166,24 +2220,37 @@ __device__ __forceinline__ void restrict_point(const RestrictGeo &G, size_t q, c
         const T dzc = T(1) / (dD + dU);
         v = (T(0.5) * dzc) * (dD * Yv(kf - 1, ifn, jfn) + dU * Yv(kf + 1, ifn, jfn)) + T(0.5) * Yv(kf, ifn, jfn);
     } else v = Yv(kf, ifn, jfn);
-    rc[q] = v;
+    return v;
+}
+// Coarse point q; also clears the coarse iterate xz (the coarse V cycle starts from zero).
+template <typename T>
+__device__ __forceinline__ void restrict_point(const RestrictGeo &G, size_t q, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
+    const size_t nc = (size_t)G.nxgc * G.nygc * G.nzgc;
+    if (q >= nc) return;
+    xz[q] = T(0);
+    rc[q] = restrict_value(G, q, rf);
 }
 template <typename T>
 __global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
-    if (solve_done(G.skip)) return;
-    restrict_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, rf, rc, xz);
+    if (!G.spec && solve_done(G.skip)) return;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)G.nxgc * G.nygc * G.nzgc) return;
+    const T v = restrict_value(G, q, rf);
+    if (G.spec && solve_done(G.skip)) return;
+    xz[q] = T(0);
+    rc[q] = v;
 }
 
 // Prolongation (MGProlongation3DExplicit.f90) of the coarse correction, added to x on interior
 // columns below the free surface.
+// Interpolated correction at fine point q < nxgf * nygf * nzgf; false where q takes none (ghost points,
+// the sigma ghost level and the surface).
 template <typename T>
-__device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T *__restrict__ xf) {
-    const size_t nf = (size_t)G.nxgf * G.nygf * G.nzgf;
-    if (q >= nf) return;
+__device__ __forceinline__ bool prolong_value(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T &out) {
     const int k = (int)(q % G.nzgf);
     const size_t col = q / G.nzgf;
     const int i = (int)(col % G.nxgf), j = (int)(col / G.nxgf);
-    if (k < 1 || k > G.nzf - 1 || i < G.gx || i > G.nxf + G.gx - 1 || j < G.gy || j > G.nyf + G.gy - 1) return;
+    if (k < 1 || k > G.nzf - 1 || i < G.gx || i > G.nxf + G.gx - 1 || j < G.gy || j > G.nyf + G.gy - 1) return false;
     const int XF = G.xc ? 2 : 1, YF = G.yc ? 2 : 1, ZF = G.zc ? 2 : 1;
     auto E0 = [&](int kk, int ii, int jj) -> T {
         if (kk < 1 || kk > G.nzf - 1 || ((kk - 1) % ZF) != 0) return T(0);
@@ -2215,12 +2282,25 @@ __device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, co
         const T dzf = T(1) / (dD + dU);
         e = (dU * EY(k - 1, i, j) + dD * EY(k + 1, i, j)) * dzf;
     } else e = EY(k, i, j);
-    xf[q] = xf[q] + e;
+    out = e;
+    return true;
+}
+template <typename T>
+__device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T *__restrict__ xf) {
+    if (q >= (size_t)G.nxgf * G.nygf * G.nzgf) return;
+    T e;
+    if (prolong_value(G, q, ec, e)) xf[q] = xf[q] + e;
 }
 template <typename T>
 __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__restrict__ xf) {
-    if (solve_done(G.skip)) return;
-    prolong_point(G, (size_t)blockIdx.x * blockDim.x + threadIdx.x, ec, xf);
+    if (!G.spec && solve_done(G.skip)) return;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)G.nxgf * G.nygf * G.nzgf) return;
+    T e;
+    if (!prolong_value(G, q, ec, e)) return;
+    const T x0 = xf[q];
+    if (G.spec && solve_done(G.skip)) return;
+    xf[q] = x0 + e;
 }
 
 // Coarsest level: x = Minv b with the dense inverse computed at set-up, stored as M (T, or FP16 with

@@ -234,6 +234,10 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         L->args.gx = g.gx; L->args.gy = g.gy; L->args.dim3 = g.ny > 1;
         L->args.sx = L->sx.p; L->args.sy = L->sy.p; L->args.sz = L->sz.p; L->args.zs = L->zs.p; L->args.col = L->col.p;
         L->args.skip = t->done();
+        // a launch on a level of up to 2^20 points is a chain of memory latencies, not a stream of
+        // bytes: its kernels issue their loads before testing skip (one wait instead of two), at the cost
+        // of reading the level once more in launches that follow convergence
+        L->args.spec = g.n <= (1u << 20) ? 1 : 0;
         mg.push_back(std::move(L));
     }
     (void)s;
@@ -360,6 +364,7 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     if (smooth(t, L, t->cfg.nu1, cur, s)) return -1;
     if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p, t->done());
+    G.spec = L.args.spec;
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
