@@ -2186,16 +2186,13 @@ struct RestrictGeo {
     const int *skip;                                 // solve_done flag (nullptr: always run)
     int spec;                                        // 1: loads are issued before skip is tested
 };
-// Restricted residual at coarse point q < nxgc * nygc * nzgc (0 on ghost points).
+// Restricted residual at coarse point (kc, ic, jc) (0 on ghost points).
 template <typename T>
-__device__ __forceinline__ T restrict_value(const RestrictGeo &G, size_t q, const T *__restrict__ rf) {
-    const int kc = (int)(q % G.nzgc);
-    const size_t col = q / G.nzgc;
-    const int ic = (int)(col % G.nxgc), jc = (int)(col / G.nxgc);
+__device__ __forceinline__ T restrict_at(const RestrictGeo &G, int kc, int ic, int jc, const T *__restrict__ rf) {
     const int icn = ic - G.gx, jcn = jc - G.gy;
     if (kc < 1 || icn < 0 || icn >= G.nxc || jcn < 0 || jcn >= G.nyc) return T(0);
     const int kf = 1 + (kc - 1) * (1 + G.zc), ifn = G.gx + icn * (1 + G.xc), jfn = G.gy + jcn * (1 + G.yc);
-    auto R = [&](int k, int i, int j) -> T { return rf[((size_t)j * G.nxgf + i) * G.nzgf + k]; };
+    auto R = [&](int k, int i, int j) -> T { return rf[(size_t)(j * G.nxgf + i) * G.nzgf + k]; };
     // x pass value at (k,i,j)
     auto Xv = [&](int k, int i, int j) -> T {
         if (G.xc && k >= 1 && k <= G.nzf - 1 && j >= G.gy && j <= G.nyf + G.gy - 1 && ((i - G.gx) % 2 == 0) && i >= G.gx && i <= G.nxf + G.gx - 1) {
@@ -2207,7 +2204,7 @@ __device__ __forceinline__ T restrict_value(const RestrictGeo &G, size_t q, cons
     };
     auto Yv = [&](int k, int i, int j) -> T {
         if (G.yc && k >= 1 && k <= G.nzf - 1 && ((j - G.gy) % 2 == 0) && j >= G.gy && j <= G.nyf + G.gy - 1 &&
-            i >= G.gx && i <= G.nxf + G.gx - 1 && ((i - G.gx) % (1 + G.xc) == 0)) {
+            i >= G.gx && i <= G.nxf + G.gx - 1 && (((i - G.gx) & G.xc) == 0)) {
             const T dS = (T)(G.yf[j] - G.yf[j - 1]), dN = (T)(G.yf[j + 1] - G.yf[j]);
             const T dyc = T(1) / (dS + dN);
             return (T(0.5) * dyc) * ((dS * Xv(k, i, j - 1) + (dN + dS) * Xv(k, i, j)) + dN * Xv(k, i, j + 1));
@@ -2221,6 +2218,13 @@ __device__ __forceinline__ T restrict_value(const RestrictGeo &G, size_t q, cons
         v = (T(0.5) * dzc) * (dD * Yv(kf - 1, ifn, jfn) + dU * Yv(kf + 1, ifn, jfn)) + T(0.5) * Yv(kf, ifn, jfn);
     } else v = Yv(kf, ifn, jfn);
     return v;
+}
+// Restricted residual at coarse point q < nxgc * nygc * nzgc.
+template <typename T>
+__device__ __forceinline__ T restrict_value(const RestrictGeo &G, size_t q, const T *__restrict__ rf) {
+    const int kc = (int)(q % G.nzgc);
+    const size_t col = q / G.nzgc;
+    return restrict_at(G, kc, (int)(col % G.nxgc), (int)(col / G.nxgc), rf);
 }
 // Coarse point q; also clears the coarse iterate xz (the coarse V cycle starts from zero).
 template <typename T>
@@ -2240,27 +2244,41 @@ __global__ void k_mg_restrict(RestrictGeo G, const T *__restrict__ rf, T *__rest
     xz[q] = T(0);
     rc[q] = v;
 }
+// The same with one grid row of coarse columns per blockIdx.y and the coarse level count NZC known at
+// compile time: the point decodes with no runtime division.
+template <typename T, int NZC>
+__global__ void k_mg_restrict_t(RestrictGeo G, const T *__restrict__ rf, T *__restrict__ rc, T *__restrict__ xz) {
+    if (!G.spec && solve_done(G.skip)) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= G.nxgc * NZC) return;
+    const int ic = t / NZC, kc = t - ic * NZC, jc = (int)blockIdx.y;
+    const T v = restrict_at(G, kc, ic, jc, rf);
+    if (G.spec && solve_done(G.skip)) return;
+    const size_t q = (size_t)jc * G.nxgc * NZC + t;
+    xz[q] = T(0);
+    rc[q] = v;
+}
 
 // Prolongation (MGProlongation3DExplicit.f90) of the coarse correction, added to x on interior
 // columns below the free surface.
 // Interpolated correction at fine point q < nxgf * nygf * nzgf; false where q takes none (ghost points,
 // the sigma ghost level and the surface).
+// (The factors XF, YF, ZF are 1 or 2: x % F is x & (F - 1) and x / F is x >> (F - 1) on the
+// non-negative offsets they are applied to.)
 template <typename T>
-__device__ __forceinline__ bool prolong_value(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T &out) {
-    const int k = (int)(q % G.nzgf);
-    const size_t col = q / G.nzgf;
-    const int i = (int)(col % G.nxgf), j = (int)(col / G.nxgf);
+__device__ __forceinline__ bool prolong_at(const RestrictGeo &G, int k, int i, int j, const T *__restrict__ ec, T &out) {
     if (k < 1 || k > G.nzf - 1 || i < G.gx || i > G.nxf + G.gx - 1 || j < G.gy || j > G.nyf + G.gy - 1) return false;
     const int XF = G.xc ? 2 : 1, YF = G.yc ? 2 : 1, ZF = G.zc ? 2 : 1;
+    const int xs = G.xc ? 1 : 0, ys = G.yc ? 1 : 0, zs = G.zc ? 1 : 0;
     auto E0 = [&](int kk, int ii, int jj) -> T {
-        if (kk < 1 || kk > G.nzf - 1 || ((kk - 1) % ZF) != 0) return T(0);
-        if (((ii - G.gx) % XF) != 0 || ((jj - G.gy) % YF) != 0) return T(0);
-        const int kc = 1 + (kk - 1) / ZF, ic = G.gx + (ii - G.gx) / XF, jc = G.gy + (jj - G.gy) / YF;
+        if (kk < 1 || kk > G.nzf - 1 || ((kk - 1) & zs) != 0) return T(0);
+        if (((ii - G.gx) & xs) != 0 || ((jj - G.gy) & ys) != 0) return T(0);
+        const int kc = 1 + ((kk - 1) >> zs), ic = G.gx + ((ii - G.gx) >> xs), jc = G.gy + ((jj - G.gy) >> ys);
         if (kc > G.nzc - 1) return T(0);
-        return ec[((size_t)jc * G.nxgc + ic) * G.nzgc + kc];
+        return ec[(size_t)(jc * G.nxgc + ic) * G.nzgc + kc];
     };
     auto EX = [&](int kk, int ii, int jj) -> T {
-        if (XF == 2 && ((jj - G.gy) % YF) == 0 && ((ii - G.gx) % 2) == 1 && kk >= 1 && kk <= G.nzf - 1 && ((kk - 1) % ZF) == 0) {
+        if (XF == 2 && ((jj - G.gy) & ys) == 0 && ((ii - G.gx) % 2) == 1 && kk >= 1 && kk <= G.nzf - 1 && ((kk - 1) & zs) == 0) {
             const T dW = (T)(G.xf[ii] - G.xf[ii - 1]), dE = (T)(G.xf[ii + 1] - G.xf[ii]);
             const T dxf = T(1) / (dW + dE);
             return (dE * E0(kk, ii - 1, jj) + dW * E0(kk, ii + 1, jj)) * dxf;
@@ -2269,7 +2287,7 @@ __device__ __forceinline__ bool prolong_value(const RestrictGeo &G, size_t q, co
     };
     auto EY = [&](int kk, int ii, int jj) -> T {
         if (kk > G.nzf - 1) return T(0);
-        if (YF == 2 && ((jj - G.gy) % 2) == 1 && kk >= 1 && ((kk - 1) % ZF) == 0) {
+        if (YF == 2 && ((jj - G.gy) % 2) == 1 && kk >= 1 && ((kk - 1) & zs) == 0) {
             const T dS = (T)(G.yf[jj] - G.yf[jj - 1]), dN = (T)(G.yf[jj + 1] - G.yf[jj]);
             const T dyf = T(1) / (dS + dN);
             return (dN * EX(kk, ii, jj - 1) + dS * EX(kk, ii, jj + 1)) * dyf;
@@ -2286,6 +2304,12 @@ __device__ __forceinline__ bool prolong_value(const RestrictGeo &G, size_t q, co
     return true;
 }
 template <typename T>
+__device__ __forceinline__ bool prolong_value(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T &out) {
+    const int k = (int)(q % G.nzgf);
+    const size_t col = q / G.nzgf;
+    return prolong_at(G, k, (int)(col % G.nxgf), (int)(col / G.nxgf), ec, out);
+}
+template <typename T>
 __device__ __forceinline__ void prolong_point(const RestrictGeo &G, size_t q, const T *__restrict__ ec, T *__restrict__ xf) {
     if (q >= (size_t)G.nxgf * G.nygf * G.nzgf) return;
     T e;
@@ -2298,6 +2322,21 @@ __global__ void k_mg_prolong_add(RestrictGeo G, const T *__restrict__ ec, T *__r
     if (q >= (size_t)G.nxgf * G.nygf * G.nzgf) return;
     T e;
     if (!prolong_value(G, q, ec, e)) return;
+    const T x0 = xf[q];
+    if (G.spec && solve_done(G.skip)) return;
+    xf[q] = x0 + e;
+}
+// The same with one grid row of fine columns per blockIdx.y and the fine level count NZF known at
+// compile time.
+template <typename T, int NZF>
+__global__ void k_mg_prolong_add_t(RestrictGeo G, const T *__restrict__ ec, T *__restrict__ xf) {
+    if (!G.spec && solve_done(G.skip)) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= G.nxgf * NZF) return;
+    const int i = t / NZF, k = t - i * NZF, j = (int)blockIdx.y;
+    T e;
+    if (!prolong_at(G, k, i, j, ec, e)) return;
+    const size_t q = (size_t)j * G.nxgf * NZF + t;
     const T x0 = xf[q];
     if (G.spec && solve_done(G.skip)) return;
     xf[q] = x0 + e;

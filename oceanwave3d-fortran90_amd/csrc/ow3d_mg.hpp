@@ -280,6 +280,29 @@ void ghost_pass(MgLevel<T> &L, T *x, hipStream_t s) {
     hipLaunchKernelGGL((k_mg_ghost_p<T>), dim3((nb + 255) / 256), dim3(256), 0, s, L.args, x, (const T *)L.b.p);
 }
 
+// Restriction and prolongation: one grid row of columns per blockIdx.y with the level count a template
+// argument (no runtime division per point) for nzg 3..14, else the flat kernels.
+template <typename T>
+void restrict_launch(const RestrictGeo &G, const T *rf, T *rc, T *xz, hipStream_t s) {
+    switch (G.nzgc) {
+#define OW3D_NZ(N) case N: hipLaunchKernelGGL((k_mg_restrict_t<T, N>), dim3((G.nxgc * N + 255) / 256, G.nygc), dim3(256), 0, s, G, rf, rc, xz); return;
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks((size_t)G.nxgc * G.nygc * G.nzgc)), dim3(256), 0, s, G, rf, rc, xz);
+}
+template <typename T>
+void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
+    switch (G.nzgf) {
+#define OW3D_NZ(N) case N: hipLaunchKernelGGL((k_mg_prolong_add_t<T, N>), dim3((G.nxgf * N + 255) / 256, G.nygf), dim3(256), 0, s, G, ec, xf); return;
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks((size_t)G.nxgf * G.nygf * G.nzgf)), dim3(256), 0, s, G, ec, xf);
+}
+
 // Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
 // under it while other blocks write their results).
 template <typename T, int NZ>
@@ -365,10 +388,10 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p, t->done());
     G.spec = L.args.spec;
-    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks(C.g.n)), dim3(256), 0, s, G, (const T *)L.r.p, C.b.p, C.x.p);
+    restrict_launch<T>(G, (const T *)L.r.p, C.b.p, C.x.p, s);
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
-    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks(L.g.n)), dim3(256), 0, s, G, (const T *)ccur, cur);
+    prolong_launch<T>(G, (const T *)ccur, cur, s);
     if (smooth(t, L, t->cfg.nu2, cur, s)) return -1;
     return 0;
 }
