@@ -66,15 +66,15 @@ struct WindowRegs {
             v[q] = (e < TOT && r < rows && c < cols * NZ) ? src[(size_t)r * pitch + c] : T(0);
         }
     }
-    template <int NZP, int RS>
-    __device__ __forceinline__ void store(T *lds, int rows, int cols) const {
+    template <int NZP, int RS, typename U>
+    __device__ __forceinline__ void store(U *lds, int rows, int cols) const {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = (int)threadIdx.x + q * NT;
             const int r = e / RL, c = e - r * RL;
             if (e < TOT && r < rows && c < cols * NZ) {
                 const int cc = c / NZ, k = c - cc * NZ;
-                lds[(r * RS + cc) * NZP + k] = v[q];
+                lds[(r * RS + cc) * NZP + k] = (U)v[q];
             }
         }
     }
@@ -1951,8 +1951,14 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // half-sweep covers the tile and reads those red values.  The wall ghost pass before the sweep is done
 // on the staged window and the one after it for the ghost points next to the tile, so one launch does
 // (ghost pass, red, black, ghost pass) with x read once and written once.
-template <typename T, int NZ, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b) {
+//
+// ZX: the first sweep of a preconditioner application, from x = 0 with the right-hand side given as the
+// FP64 residual r0: x is not read, b = (T) r0 is staged from r0 and written to bout for the points the
+// tile owns (its columns and the ghost points the post-sweep ghost pass gives it), which together cover
+// the level; this replaces the separate pass b = (T) r, x = 0 before the V cycle.
+template <typename T, int NZ, int TX, int TY, int NT, bool ZX = false>
+__global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b,
+                                                    const double *__restrict__ r0, T *__restrict__ bout) {
     if (!L.spec && solve_done(L.skip)) return;
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
@@ -1977,16 +1983,34 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     const int bcols = cmin(ie + 1, nxg) - blox, brows = d3 ? cmin(je + 1, nyg) - bloy : 1;
     // every HBM load of the block (coefficients, x window, b ring) is in flight before the first wait
     CoefRegs<T, NZ, WX, WY, NT> cr;
-    WindowRegs<T, NZ, WY, WX, NT> xr;
-    WindowRegs<T, NZ, TY + 2, TX + 2, NT> br;
+    WindowRegs<T, NZ, WY, ZX ? 1 : WX, NT> xr;   // unused when ZX
+    WindowRegs<typename std::conditional<ZX, double, T>::type, NZ, TY + 2, TX + 2, NT> br;
     cr.load(L, lox, loy, wcols, wrows);
-    xr.load(xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
-    br.load(b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
+    if constexpr (ZX) br.load(r0 + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
+    else {
+        xr.load(xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+        br.load(b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
+    }
     if (L.spec && solve_done(L.skip)) return;
     const CoefS<T> C = cr.store(sC);
-    xr.template store<NZP, WX>(sX, wrows, wcols);
+    if constexpr (ZX) {
+        for (int e = threadIdx.x; e < WX * WY * NZP; e += NT) sX[e] = T(0);
+    } else {
+        xr.template store<NZP, WX>(sX, wrows, wcols);
+    }
     br.template store<NZ, TX + 2>(sB, brows, bcols);
     __syncthreads();
+    // owned points [gi0, gi1) x [gj0, gj1): the tile's columns and the ghost points beside them
+    const int gi0 = (i0 == L.gx) ? 0 : i0, gi1 = (ie == nxg - L.gx) ? nxg : ie;
+    const int gj0 = d3 ? ((j0 == L.gy) ? 0 : j0) : 0, gj1 = d3 ? ((je == nyg - L.gy) ? nyg : je) : 1;
+    if constexpr (ZX) {
+        const int w = (gi1 - gi0) * NZ;
+        for (int r = gj0; r < gj1; r++) {
+            T *dst = bout + ((size_t)r * nxg + gi0) * NZ;
+            const T *src = sB + ((r - bloy) * (TX + 2) + (gi0 - blox)) * NZ;
+            for (int e = threadIdx.x; e < w; e += NT) dst[e] = src[e];
+        }
+    }
     auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
     const int jlo = d3 ? L.gy : 0, jhi = d3 ? nyg - L.gy : 1;
     // value of wall ghost point (ii, jj, k) of the ring-extended tile from the staged interior values
@@ -2092,8 +2116,6 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     // the ghost pass after the sweep for the ghost points this tile owns: wall columns beside its rows,
     // wall rows beside its columns, and the corners of the tiles at the corners of the domain
     {
-        const int gi0 = (i0 == L.gx) ? 0 : i0, gi1 = (ie == nxg - L.gx) ? nxg : ie;
-        const int gj0 = d3 ? ((j0 == L.gy) ? 0 : j0) : 0, gj1 = d3 ? ((je == nyg - L.gy) ? nyg : je) : 1;
         if (gi0 != i0 || gi1 != ie || gj0 != j0 || gj1 != je) {
             const int w = gi1 - gi0;
             for (int e = threadIdx.x; e < (gj1 - gj0) * w * NZ; e += NT) {

@@ -306,7 +306,8 @@ void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
 // Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
 // under it while other blocks write their results).
 template <typename T, int NZ>
-void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s) {
+void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
+                  const double *r0, T *bout) {
     // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
@@ -316,14 +317,22 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int thr
     // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
     const bool big = threads == 1024 || (threads == 0 && (long)grid.x * grid.y < 256);
-    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b);
-    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b);
+    if (r0) {   // first sweep from x = 0, b = (T) r0 written to bout
+        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, true>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout);
+        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, true>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout);
+        return;
+    }
+    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout);
+    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout);
 }
 
+// Fused sweep xin -> xout; with r0 (not NULL) the first sweep of an application: x = 0 and b = (T) r0,
+// written to bout (the level's b).
 template <typename T>
-int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s) {
+int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
+              const double *r0 = nullptr, T *bout = nullptr) {
     switch (L.nzg) {
-#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s); return 0;
+#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s, r0, bout); return 0;
         OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
         OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
 #undef OW3D_NZ
@@ -333,13 +342,15 @@ int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads
 
 // ghost pass, nu x (zebra sweep, ghost pass) on the level's current iterate `cur` (L.x or L.x2; a fused
 // sweep moves it to the other buffer).
+// r0 (fused sweeps, nu >= 1 only): the level's x is zero and its b is (T) r0, both still to be set; the
+// first sweep does it.
 template <typename T>
-int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s) {
+int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s, const double *r0 = nullptr) {
     if (t->fused_rb) {   // each launch is (ghost pass, red, black, ghost pass)
         if (nu == 0) ghost_pass(L, cur, s);
         for (int it = 0; it < nu; it++) {
             T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
-            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s)) return -1;
+            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s, it == 0 ? r0 : nullptr, L.b.p)) return -1;
             cur = nxt;
         }
         return 0;
@@ -360,7 +371,8 @@ template <> inline const BottomArgs<float> *bottom_args<float>(const ow3d_tank *
 // V cycle on level k from its current iterate `cur` (on entry L.x for every level below the finest);
 // on return `cur` points at the buffer holding the result.
 template <typename T>
-int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T *&cur, hipStream_t s) {
+int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T *&cur, hipStream_t s,
+           const double *r0 = nullptr) {
     MgLevel<T> &L = *mg[k];
     if (k == t->bot_top) {   // in place on L.x
         switch (L.g.nzg) {
@@ -384,7 +396,7 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
         return 0;
     }
     MgLevel<T> &C = *mg[k - 1];
-    if (smooth(t, L, t->cfg.nu1, cur, s)) return -1;
+    if (smooth(t, L, t->cfg.nu1, cur, s, r0)) return -1;
     if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p, t->done());
     G.spec = L.args.spec;
@@ -404,11 +416,15 @@ int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, con
     const size_t n = t->g.n;
     const int top = t->nlev - 1;
     MgLevel<T> &F = *mg[top];
-    hipLaunchKernelGGL((k_precond_in<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, F.x.p, t->done());
+    // with fused sweeps and pre-smoothing above the coarsest level, the first sweep sets b = (T) r from
+    // x = 0 itself; otherwise one pass does it first
+    const bool fuse_in = t->fused_rb && t->cfg.nu1 >= 1 && top >= 1 && top != t->bot_top;
+    if (!fuse_in)
+        hipLaunchKernelGGL((k_precond_in<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, F.x.p, t->done());
     T *cur = F.x.p;
     for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
         if (t->nlev == 1 && it > 0) break;
-        if (vcycle<T>(t, mg, top, cur, s)) return -1;
+        if (vcycle<T>(t, mg, top, cur, s, (fuse_in && it == 0) ? r : nullptr)) return -1;
     }
     *out = cur;
     return 0;
