@@ -178,6 +178,6 @@ int mg_build64(ow3d_tank *t);
 int mg_build32(ow3d_tank *t);
 int mg_fill_bottom64(ow3d_tank *t);
 int mg_fill_bottom32(ow3d_tank *t);
-int precond_body64(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
-int precond_body32(ow3d_tank *t, const double *r, const void **out, hipStream_t s);
+int precond_body64(ow3d_tank *t, const double *r, const void **out, hipStream_t s, double *u);
+int precond_body32(ow3d_tank *t, const double *r, const void **out, hipStream_t s, double *u);
 }  // namespace ow3d_host

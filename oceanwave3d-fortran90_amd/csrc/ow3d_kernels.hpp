@@ -1952,13 +1952,16 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // on the staged window and the one after it for the ghost points next to the tile, so one launch does
 // (ghost pass, red, black, ghost pass) with x read once and written once.
 //
-// ZX: the first sweep of a preconditioner application, from x = 0 with the right-hand side given as the
-// FP64 residual r0: x is not read, b = (T) r0 is staged from r0 and written to bout for the points the
+// MODE 1: the first sweep of a preconditioner application, from x = 0 with the right-hand side given as
+// the FP64 residual r0: x is not read, b = (T) r0 is staged from r0 and written to bout for the points the
 // tile owns (its columns and the ghost points the post-sweep ghost pass gives it), which together cover
 // the level; this replaces the separate pass b = (T) r, x = 0 before the V cycle.
-template <typename T, int NZ, int TX, int TY, int NT, bool ZX = false>
+// MODE 2: the last sweep of an application: the owned points of the result go to uout as FP64 (the
+// preconditioner's output) instead of to xout; this replaces the separate output conversion.
+template <typename T, int NZ, int TX, int TY, int NT, int MODE = 0>
 __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b,
-                                                    const double *__restrict__ r0, T *__restrict__ bout) {
+                                                    const double *__restrict__ r0, T *__restrict__ bout, double *__restrict__ uout) {
+    constexpr bool ZX = MODE == 1, ZO = MODE == 2;
     if (!L.spec && solve_done(L.skip)) return;
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
@@ -2106,11 +2109,13 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     };
     half(std::integral_constant<int, 0>{});
     half(std::integral_constant<int, 1>{});
+    using O = typename std::conditional<ZO, double, T>::type;
+    O *const out = ZO ? (O *)uout : (O *)xout;
     for (int r = 0; r < je - j0; r++) {
-        T *dst = xout + ((size_t)(j0 + r) * nxg + i0) * NZ;
+        O *dst = out + ((size_t)(j0 + r) * nxg + i0) * NZ;
         for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
             const int c = e / NZ, k = e - c * NZ;
-            dst[e] = sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
+            dst[e] = (O)sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
         }
     }
     // the ghost pass after the sweep for the ghost points this tile owns: wall columns beside its rows,
@@ -2121,7 +2126,7 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
             for (int e = threadIdx.x; e < (gj1 - gj0) * w * NZ; e += NT) {
                 const int r = e / (w * NZ), rem = e - r * w * NZ, c = rem / NZ, k = rem - c * NZ;
                 const int ii = gi0 + c, jj = gj0 + r;
-                if (is_ghost(ii, jj)) xout[((size_t)jj * nxg + ii) * NZ + k] = ghost_value(ii, jj, k);
+                if (is_ghost(ii, jj)) out[((size_t)jj * nxg + ii) * NZ + k] = (O)ghost_value(ii, jj, k);
             }
         }
     }

@@ -46,7 +46,8 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
             std::lock_guard<std::mutex> lk(capture_lock());
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s));
+            int rc = (t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s, nullptr)
+                                               : precond_body32(t, r, &t->mg_out, s, nullptr));
             hipError_t e = hipStreamEndCapture(s, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return -1; }
             HIPCHK(e);
@@ -55,7 +56,11 @@ int precond(ow3d_tank *t, const double *r, double *u, hipStream_t s) {
         }
         HIPCHK(hipGraphLaunch(t->pg_exec, s));
     } else {
-        if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, &t->mg_out, s) : precond_body32(t, r, &t->mg_out, s))) return -1;
+        // inline (eager, or inside a captured step): the last sweep writes u itself
+        const void *o = nullptr;
+        if ((t->cfg.mg_precision == 0 ? precond_body64(t, r, &o, s, u) : precond_body32(t, r, &o, s, u))) return -1;
+        if (!o) return 0;
+        t->mg_out = o;
     }
     if (t->cfg.mg_precision == 0)
         hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_blocks(n)), dim3(256), 0, s, n, (const double *)t->mg_out, u, t->done());

@@ -307,7 +307,7 @@ void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
 // under it while other blocks write their results).
 template <typename T, int NZ>
 void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
-                  const double *r0, T *bout) {
+                  const double *r0, T *bout, double *uout) {
     // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
@@ -318,21 +318,27 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int thr
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
     const bool big = threads == 1024 || (threads == 0 && (long)grid.x * grid.y < 256);
     if (r0) {   // first sweep from x = 0, b = (T) r0 written to bout
-        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, true>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout);
-        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, true>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout);
+        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, 1>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
+        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, 1>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
         return;
     }
-    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout);
-    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout);
+    if (uout) {   // last sweep: the result goes to uout in FP64
+        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, 2>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
+        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, 2>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
+        return;
+    }
+    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
+    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
 }
 
 // Fused sweep xin -> xout; with r0 (not NULL) the first sweep of an application: x = 0 and b = (T) r0,
-// written to bout (the level's b).
+// written to bout (the level's b); with uout (not NULL) the last: the result goes to uout in FP64 and
+// xout is not written.
 template <typename T>
 int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
-              const double *r0 = nullptr, T *bout = nullptr) {
+              const double *r0 = nullptr, T *bout = nullptr, double *uout = nullptr) {
     switch (L.nzg) {
-#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s, r0, bout); return 0;
+#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s, r0, bout, uout); return 0;
         OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
         OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
 #undef OW3D_NZ
@@ -343,14 +349,17 @@ int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads
 // ghost pass, nu x (zebra sweep, ghost pass) on the level's current iterate `cur` (L.x or L.x2; a fused
 // sweep moves it to the other buffer).
 // r0 (fused sweeps, nu >= 1 only): the level's x is zero and its b is (T) r0, both still to be set; the
-// first sweep does it.
+// first sweep does it.  uout (fused sweeps, nu >= 1 only): the last sweep writes its result there in FP64
+// (cur is then stale).
 template <typename T>
-int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s, const double *r0 = nullptr) {
+int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s, const double *r0 = nullptr,
+           double *uout = nullptr) {
     if (t->fused_rb) {   // each launch is (ghost pass, red, black, ghost pass)
         if (nu == 0) ghost_pass(L, cur, s);
         for (int it = 0; it < nu; it++) {
             T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
-            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s, it == 0 ? r0 : nullptr, L.b.p)) return -1;
+            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s, it == 0 ? r0 : nullptr, L.b.p,
+                             it == nu - 1 ? uout : nullptr)) return -1;
             cur = nxt;
         }
         return 0;
@@ -372,7 +381,7 @@ template <> inline const BottomArgs<float> *bottom_args<float>(const ow3d_tank *
 // on return `cur` points at the buffer holding the result.
 template <typename T>
 int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T *&cur, hipStream_t s,
-           const double *r0 = nullptr) {
+           const double *r0 = nullptr, double *uout = nullptr) {
     MgLevel<T> &L = *mg[k];
     if (k == t->bot_top) {   // in place on L.x
         switch (L.g.nzg) {
@@ -404,29 +413,33 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
     prolong_launch<T>(G, (const T *)ccur, cur, s);
-    if (smooth(t, L, t->cfg.nu2, cur, s)) return -1;
+    if (smooth(t, L, t->cfg.nu2, cur, s, nullptr, uout)) return -1;
     return 0;
 }
 
 // The preconditioner body in precision T: input r (FP64) -> finest-level right-hand side, V cycles from
 // x = 0.  Only kernel launches (no memset / memcpy nodes), so the sequence can be captured in a hipGraph.
 // Returns the buffer holding the result in *out.
+// With u (not NULL) and post-smoothing on the finest level, the last sweep writes the result to u in FP64
+// and *out is set to NULL; otherwise *out is the finest-level buffer holding it.
 template <typename T>
-int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, const double *r, const void **out, hipStream_t s) {
+int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, const double *r, const void **out, hipStream_t s,
+                 double *u) {
     const size_t n = t->g.n;
     const int top = t->nlev - 1;
     MgLevel<T> &F = *mg[top];
     // with fused sweeps and pre-smoothing above the coarsest level, the first sweep sets b = (T) r from
     // x = 0 itself; otherwise one pass does it first
-    const bool fuse_in = t->fused_rb && t->cfg.nu1 >= 1 && top >= 1 && top != t->bot_top;
+    const bool fused = t->fused_rb && top >= 1 && top != t->bot_top;
+    const bool fuse_in = fused && t->cfg.nu1 >= 1, fuse_out = fused && u && t->cfg.nu2 >= 1;
     if (!fuse_in)
         hipLaunchKernelGGL((k_precond_in<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, F.x.p, t->done());
     T *cur = F.x.p;
-    for (int it = 0; it < std::max(1, t->cfg.maxit); it++) {
-        if (t->nlev == 1 && it > 0) break;
-        if (vcycle<T>(t, mg, top, cur, s, (fuse_in && it == 0) ? r : nullptr)) return -1;
+    const int cycles = t->nlev == 1 ? 1 : std::max(1, t->cfg.maxit);
+    for (int it = 0; it < cycles; it++) {
+        if (vcycle<T>(t, mg, top, cur, s, (fuse_in && it == 0) ? r : nullptr, (fuse_out && it == cycles - 1) ? u : nullptr)) return -1;
     }
-    *out = cur;
+    *out = fuse_out ? nullptr : cur;
     return 0;
 }
 
