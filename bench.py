@@ -222,9 +222,10 @@ def main():
 
 def bench_batched_513(ow3d, problems, k, steps, warmup):
     """k tanks of the 513x513x9 workload (JONSWAP seeds 0 .. k-1) advanced together with
-    ow3d.batch_rk4_steps, FP64 and FP32 multigrid: DOF-steps/s of the whole batch over `steps` steps in one
-    call (each tank starts its next step as soon as its previous one has settled), then the latency of a
-    step of the whole batch (one call per step, so every step waits for the slowest tank)."""
+    ow3d.batch_rk4_steps, FP64 and FP32 multigrid: the latency of a step of the whole batch (one call per
+    step, so every step waits for the slowest tank), then DOF-steps/s of the whole batch over `steps` more
+    steps in one call (each tank starts its next step as soon as its previous one has settled; the stage
+    graphs the step counts need are captured by then)."""
     out = {"tanks": k}
     for label, prec in (("fp64", 0), ("mixed", 1)):
         tanks, dts = [], []
@@ -232,14 +233,14 @@ def bench_batched_513(ow3d, problems, k, steps, warmup):
             T, cfg, dt = make_tank(ow3d, problems, prec, seed=s)
             tanks.append(T); dts.append(dt)
         ow3d.batch_rk4_steps(tanks, warmup, dts)
-        a = time.perf_counter()
-        ow3d.batch_rk4_steps(tanks, steps, dts)
-        t_all = time.perf_counter() - a
         lat = []
         for _ in range(steps):
             a = time.perf_counter()
             ow3d.batch_rk4_steps(tanks, 1, dts)
             lat.append(time.perf_counter() - a)
+        a = time.perf_counter()
+        ow3d.batch_rk4_steps(tanks, steps, dts)
+        t_all = time.perf_counter() - a
         dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1) * k
         out[label] = {"dof_steps_per_s": dof * steps / t_all, "ms_per_batch_step": 1e3 * t_all / steps,
                       "dof_steps_per_s_stepwise": dof * steps / sum(lat),
