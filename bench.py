@@ -108,7 +108,7 @@ def main():
                     help="only the --batched-tanks run of the 513x513x9 tanks (prints its own JSON line)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="many-tank workload instead: N uneven tanks advanced together (prints its own JSON line)")
-    ap.add_argument("--untuned", action="store_true", help="--batch without the autotuner's cached schedules")
+    ap.add_argument("--untuned", action="store_true", help="--batch / --serve without the autotuner's cached schedules")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
                     help="request-loop workload instead: Poisson arrivals served for SECONDS (prints its own JSON line)")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
@@ -131,7 +131,7 @@ def main():
               flush=True)
         return
     if args.serve:
-        print(json.dumps(bench_serve(args.serve)), flush=True)
+        print(json.dumps(bench_serve(args.serve, tuned=not args.untuned)), flush=True)
         return
     if args.batched_only:
         print(json.dumps(bench_batched_513(ow3d, problems, args.batched_tanks, args.steps, args.warmup)), flush=True)
@@ -313,14 +313,15 @@ def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0, use_tuning=True):
             "tanks_with_unconverged_solves": int(len(conv) - sum(conv))}
 
 
-def bench_serve(seconds, rate=20.0, max_active=32, seed=0):
+def bench_serve(seconds, rate=20.0, max_active=32, seed=0, tuned=True):
     """Request-loop workload: Poisson arrivals (rate per second) of many-tank draws with 50..500 steps and
     a flat or shoal bottom, admitted by a 200 GB HBM budget and at most max_active tanks at once, served
     for `seconds` with the FP64 and then the FP32 multigrid."""
     from ow3d import serve
-    out = {"workload": f"Poisson {rate}/s tank requests, 50..500 steps, {max_active} slots, 200 GB budget, {seconds} s"}
+    out = {"workload": f"Poisson {rate}/s tank requests, 50..500 steps, {max_active} slots, 200 GB budget, {seconds} s",
+           "schedules": "tuned" if tuned else "untuned"}
     for label, prec in (("fp64", 0), ("mixed", 1)):
-        reqs = serve.poisson_requests(int(rate * seconds * 2) + 8, rate, seed, mg_precision=prec)
+        reqs = serve.poisson_requests(int(rate * seconds * 2) + 8, rate, seed, mg_precision=prec, tuned=tuned)
         out[label] = serve.Server(hbm_budget=200e9, max_active=max_active).run(reqs, time_limit=seconds)
         print(f"serve {label}: {out[label]['steps']} steps, {out[label]['completed']} requests completed",
               file=sys.stderr, flush=True)

@@ -19,6 +19,11 @@ std::mutex &ow3d_host::capture_lock() {
     return m;
 }
 
+std::unique_lock<std::mutex> *&ow3d_host::create_lock() {
+    static thread_local std::unique_lock<std::mutex> *l = nullptr;
+    return l;
+}
+
 std::atomic<long long> &ow3d_host::device_bytes() {
     static std::atomic<long long> b{0};
     return b;
@@ -188,7 +193,11 @@ __global__ void k_extrap3(size_t n, const double *__restrict__ Lnew, const doubl
 extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double *hx, const double *hy, const double *hxx,
                            const double *hyy, ow3d_tank **out) {
     if (!cfg || !h || !out) { err() = "null argument"; return -1; }
-    std::lock_guard<std::mutex> lk(capture_lock());
+    std::unique_lock<std::mutex> lk(capture_lock());
+    struct Published {   // create_lock() names lk for the duration of this call
+        explicit Published(std::unique_lock<std::mutex> *l) { create_lock() = l; }
+        ~Published() { create_lock() = nullptr; }
+    } published(&lk);
     *out = nullptr;
     const ow3d_config &c = *cfg;
     if (c.nx < 2 || c.ny < 1 || c.nz < 3) { err() = "need Nx >= 2, Ny >= 1, Nz >= 3"; return -1; }

@@ -46,6 +46,10 @@ std::atomic<long long> &device_bytes();
 // destroyed or has state copied in or out (allocation, frees and blocking copies on other threads must
 // not run during a capture).
 std::mutex &capture_lock();
+// The capture lock as held by ow3d_create on this thread (nullptr elsewhere): set-up releases it around
+// pure host work (the coarsest level's dense inverse), during which it makes no HIP call, so other
+// handles can capture meanwhile.
+std::unique_lock<std::mutex> *&create_lock();
 
 template <typename T>
 struct DevBuf {

@@ -214,13 +214,18 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         HIPCHK(hipMemcpy(L->zf.p, g.z.data(), g.nzg * 8, hipMemcpyHostToDevice));
         if (k == 0) {
             int n = (int)g.n;
-            std::vector<double> A((size_t)n * n), e(n, 0.0), colv;
-            for (int c = 0; c < n; c++) {
-                e[c] = 1.0; low_apply_host(g, e, colv); e[c] = 0.0;
-                for (int r = 0; r < n; r++) A[(size_t)r * n + c] = colv[r];
-            }
             std::vector<double> Ainv;
-            dense_inverse(A, n, Ainv);
+            {   // host work only: the capture lock is released meanwhile (see create_lock)
+                std::unique_lock<std::mutex> *cl = create_lock();
+                if (cl) cl->unlock();
+                std::vector<double> A((size_t)n * n), e(n, 0.0), colv;
+                for (int c = 0; c < n; c++) {
+                    e[c] = 1.0; low_apply_host(g, e, colv); e[c] = 0.0;
+                    for (int r = 0; r < n; r++) A[(size_t)r * n + c] = colv[r];
+                }
+                dense_inverse(A, n, Ainv);
+                if (cl) cl->lock();
+            }
             if (t->cfg.mg_precision == 2) {   // FP16 storage of the coarsest level's inverse
                 std::vector<__half> h(Ainv.size());
                 for (size_t q = 0; q < Ainv.size(); q++) h[q] = __float2half((float)Ainv[q]);

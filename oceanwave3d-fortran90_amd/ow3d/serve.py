@@ -33,9 +33,10 @@ _TABLE = autotune.load()
 
 class Request:
     """One tank simulation: a problems.batch_draw(...) dict (the tank's fields are built when it is
-    admitted), nsteps RK4 steps, multigrid precision (0 FP64, 1 FP32)."""
+    admitted), nsteps RK4 steps, multigrid precision (0 FP64, 1 FP32); tuned: use the autotuner's cached
+    schedule for the tank's class (else the default levels and launch choices)."""
 
-    def __init__(self, rid, draw, nsteps, mg_precision, arrival=0.0):
+    def __init__(self, rid, draw, nsteps, mg_precision, arrival=0.0, tuned=True):
         self.rid = rid
         self.draw = draw
         self.nsteps = int(nsteps)
@@ -43,7 +44,7 @@ class Request:
         self.arrival = float(arrival)
         cfg = problems.batch_cfg(draw)
         cfg.update(mg_precision=self.mg_precision, gmres_m=23, reltol=1e-8, abstol=0.0)
-        self.choice = autotune.tuned(cfg, _TABLE)   # the autotuner's schedule for the class, if cached
+        self.choice = autotune.tuned(cfg, _TABLE) if tuned else None   # the class's cached schedule
         if self.choice:
             cfg["mg_maxgrids"] = self.choice["mg_maxgrids"]
         self.cfg = cfg
@@ -51,7 +52,7 @@ class Request:
         self.bytes = ow3d.estimate_bytes(cfg)
 
 
-def poisson_requests(n, rate, seed=0, mg_precision=1, steps=(50, 500), sizes=problems.BATCH_SIZES):
+def poisson_requests(n, rate, seed=0, mg_precision=1, steps=(50, 500), sizes=problems.BATCH_SIZES, tuned=True):
     """n requests with exponential inter-arrival times (mean 1/rate s), tanks drawn as the many-tank
     workload (flat or Gaussian-shoal bottom at random), step counts uniform in [steps[0], steps[1]]."""
     rng = np.random.default_rng(seed)
@@ -61,7 +62,7 @@ def poisson_requests(n, rate, seed=0, mg_precision=1, steps=(50, 500), sizes=pro
         t += float(rng.exponential(1.0 / rate))
         bottom = "shoal" if rng.uniform() < 0.5 else "flat"
         draw = problems.batch_draw(int(seed) * 1000003 + i, bottom=bottom, sizes=sizes)
-        out.append(Request(i, draw, int(rng.integers(steps[0], steps[1] + 1)), mg_precision, t))
+        out.append(Request(i, draw, int(rng.integers(steps[0], steps[1] + 1)), mg_precision, t, tuned))
     return out
 
 
@@ -111,6 +112,7 @@ class Server:
         waiting = collections.deque(sorted(requests, key=lambda r: r.arrival))
         built = queue.Queue()
         want = queue.Queue()
+        setup = []   # host set-up seconds of each built tank
         stop = threading.Event()
 
         def builder():   # host set-up of admitted requests, off the loop's thread
@@ -120,7 +122,10 @@ class Server:
                 except queue.Empty:
                     continue
                 try:
-                    built.put((req, build_tank(req), None))
+                    a = time.perf_counter()
+                    tk = build_tank(req)
+                    setup.append(time.perf_counter() - a)
+                    built.put((req, tk, None))
                 except Exception as e:   # reported by the loop
                     built.put((req, None, e))
 
@@ -207,6 +212,8 @@ class Server:
                               "p99": float(np.percentile(qdelay, 99)) if qdelay else 0.0},
             "max_active": self.max_active,
             "hbm_budget_GB": self.hbm_budget / 1e9,
+            "setup_s": {"tanks": len(setup), "p50": float(np.percentile(setup, 50)) if setup else 0.0,
+                        "max": float(max(setup)) if setup else 0.0, "total": float(sum(setup))},
         }
         if self.keep_results:
             out["results"] = results
