@@ -13,7 +13,8 @@
  *
  * One handle per host thread at a time; each handle owns its device buffers and its HIP stream.
  * Handles may be created, destroyed and have state copied on one thread while others step theirs:
- * graph capture, creation, destruction and state copies are serialised by a process-wide lock.
+ * graph capture, creation, destruction and state copies are serialised by a process-wide lock (creation
+ * releases it while it computes the coarsest multigrid level's dense inverse on the host).
  */
 #ifndef OW3D_H
 #define OW3D_H
