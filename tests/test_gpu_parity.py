@@ -5,6 +5,8 @@ Tolerances (FP64 path): matvec / plane operators / vertical velocity / FSBC rhs 
 zebra-line V cycle; converged Laplace solves <= 1e-9 relative; RK4 trajectories eta and Phi after N steps
 <= 1e-10 relative when both sides converge each solve to 1e-13.  FP32 multigrid: <= 1e-4 relative on M^-1 r.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -154,6 +156,35 @@ def test_precond_deterministic_on_a_large_grid(prec):
     r0 = np.linalg.norm(b)
     assert np.linalg.norm(b - T.laplace_apply(x)) < (5e-5 if prec == 0 else 1e-3) * r0
     T.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_inline_and_graph_preconditioner_agree_bitwise(prec):
+    """An application enqueued inline (every captured RK4 stage; here a handle made with graphs off) has
+    its last zebra sweep write the FP64 result itself; one replayed from the handle's preconditioner graph
+    converts the result in a separate pass.  Both start with the sweep that stages b = (T) r from x = 0.
+    They must give the same bits.  385 x 385 x 9: the finest level (1.5 M points) tests the convergence
+    flag before loading, the smaller levels load first."""
+    ow3d = _ow3d()
+    from ow3d import problems
+    nx = ny = 385
+    bath = problems.gaussian_shoal(nx, ny, 50.0, 50.0)
+    cfg = dict(nx=nx, ny=ny, nz=9, lx=50.0, ly=50.0, mg_precision=prec, reltol=1e-8, gmres_m=23)
+    T_graph = ow3d.Tank(cfg, **bath)
+    os.environ["OW3D_NO_GRAPH"] = "1"
+    try:
+        T_inline = ow3d.Tank(cfg, **bath)
+    finally:
+        del os.environ["OW3D_NO_GRAPH"]
+    assert T_graph.mg_levels >= 3 and T_graph.n > (1 << 20)
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        r = rng.standard_normal(T_graph.shape3)
+        ug, ui = T_graph.precond_apply(r), T_inline.precond_apply(r)
+        assert np.array_equal(ug, ui)
+        assert np.all(np.isfinite(ug)) and np.max(np.abs(ug)) > 0
+    T_graph.close()
+    T_inline.close()
 
 
 @pytest.mark.parametrize("solver", [1, 0])
