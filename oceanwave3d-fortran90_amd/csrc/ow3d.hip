@@ -810,7 +810,7 @@ extern "C" long long ow3d_estimate_bytes(const ow3d_config *cfg) {
         for (size_t k = 0; k < lev.size(); k++) {
             const Grid &L = lev[k];
             b += p * ((long long)L.n * 4 + 6LL * L.nxy + 12LL * (L.nxg + L.nyg + L.nzg)) + 8LL * (L.nxg + L.nyg + L.nzg);
-            if (k == 0) b += pc * (long long)L.n * (long long)L.n;
+            if (k == 0) b += pc * (long long)L.n * (long long)minv_pitch((int)L.n);
         }
     }
     return b;

@@ -46,6 +46,8 @@ std::atomic<long long> &device_bytes();
 // destroyed or has state copied in or out (allocation, frees and blocking copies on other threads must
 // not run during a capture).
 std::mutex &capture_lock();
+// Row pitch (elements) of the coarsest level's stored inverse: n rounded up to a multiple of 8.
+inline int minv_pitch(int n) { return (n + 7) & ~7; }
 // The capture lock as held by ow3d_create on this thread (nullptr elsewhere): set-up releases it around
 // pure host work (the coarsest level's dense inverse), during which it makes no HIP call, so other
 // handles can capture meanwhile.
@@ -79,6 +81,7 @@ struct MgLevel {
     Grid g;
     DevBuf<T> sx, sy, sz, zs, col, x, x2, b, r, minv;   // x2: second iterate buffer of out-of-place sweeps
     DevBuf<__half> minv16;                             // coarsest-level inverse in FP16 (mg_precision 2)
+    int minv_pitch = 0;                                // row pitch (elements) of minv / minv16
     DevBuf<double> xf, yf, zf;   // level coordinates (restriction / prolongation weights)
     LvArgs<T> args{};
 };

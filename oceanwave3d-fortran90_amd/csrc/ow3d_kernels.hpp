@@ -2370,17 +2370,30 @@ __global__ void k_mg_prolong_add_t(RestrictGeo G, const T *__restrict__ ec, T *_
 }
 
 // Coarsest level: x = Minv b with the dense inverse computed at set-up, stored as M (T, or FP16 with
-// sums in T).
+// sums in T) in rows of `pitch` elements (a multiple of 8; zeros past n); b holds pitch elements (zeros
+// past n).  One 64-lane wave per row: each lane reads 16 bytes of the row and the matching b values per
+// step (vector loads), sums those products in order, and the lanes' sums are reduced in a fixed order.
 template <typename T, typename M>
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, const M *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x,
-                                                   const int *skip) {
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, int pitch, const M *__restrict__ Minv, const T *__restrict__ b,
+                                                   T *__restrict__ x, const int *skip) {
     if (solve_done(skip)) return;
-    // one 64-lane wave per row: coalesced reads of the row, fixed-order lane sums and wave reduction
+    constexpr int V = 16 / (int)sizeof(M);
     const int lane = threadIdx.x & 63, r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n) return;
-    const M *row = Minv + (size_t)r * n;
+    const M *row = Minv + (size_t)r * pitch;
     T s = T(0);
-    for (int c = lane; c < n; c += 64) s += (T)row[c] * b[c];
+#pragma unroll 4
+    for (int c0 = lane * V; c0 < n; c0 += 64 * V) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(row + c0);
+        M m[V];
+        __builtin_memcpy(m, &q, 16);
+        T bv[V];
+        __builtin_memcpy(bv, __builtin_assume_aligned(b + c0, 16), V * sizeof(T));
+        T p = T(0);
+#pragma unroll
+        for (int v = 0; v < V; v++) p += (T)m[v] * bv[v];
+        s += p;
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) x[r] = s;
@@ -2396,7 +2409,7 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const M *__restrict__ 
 constexpr int MG_BOTTOM_MAX = 10;
 template <typename T>
 struct BottomArgs {
-    int nlev, nu1, nu2, n0;
+    int nlev, nu1, nu2, n0, pitch0;   // pitch0: row pitch (elements) of minv
     const T *minv;
     LvArgs<T> L[MG_BOTTOM_MAX];
     T *x[MG_BOTTOM_MAX];
@@ -2479,7 +2492,7 @@ __global__ __launch_bounds__(512) void k_mg_bottom(const BottomArgs<T> *__restri
         T *x0 = A->x[0];
         for (int rr = threadIdx.x; rr < n0; rr += blockDim.x) {
             T s = T(0);
-            for (int c = 0; c < n0; c++) s += minv[(size_t)rr * n0 + c] * b0[c];
+            for (int c = 0; c < n0; c++) s += minv[(size_t)rr * A->pitch0 + c] * b0[c];
             x0[rr] = s;
         }
     }

@@ -207,7 +207,10 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
             col[5 * g.nxy + p] = -(2.0 * (hx * hx + hy * hy) / (d * d)) + hs / d;
         }
         HIPCHK(up(L->sx, g.sx)); HIPCHK(up(L->sy, sy)); HIPCHK(up(L->sz, g.sz)); HIPCHK(up(L->zs, zs)); HIPCHK(up(L->col, col));
-        HIPCHK(L->x.alloc(g.n)); HIPCHK(L->x2.alloc(g.n)); HIPCHK(L->b.alloc(g.n)); HIPCHK(L->r.alloc(g.n));
+        // the coarsest level's b is padded with zeros to the inverse's row pitch (whole 16-byte loads)
+        const size_t nb = k == 0 ? (size_t)minv_pitch((int)g.n) : g.n;
+        HIPCHK(L->x.alloc(g.n)); HIPCHK(L->x2.alloc(g.n)); HIPCHK(L->b.alloc(nb)); HIPCHK(L->r.alloc(g.n));
+        if (k == 0) HIPCHK(hipMemset(L->b.p, 0, nb * sizeof(T)));
         HIPCHK(L->xf.alloc(g.nxg)); HIPCHK(L->yf.alloc(g.nyg)); HIPCHK(L->zf.alloc(g.nzg));
         HIPCHK(hipMemcpy(L->xf.p, g.x.data(), g.nxg * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(L->yf.p, g.y.data(), g.nyg * 8, hipMemcpyHostToDevice));
@@ -226,13 +229,19 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
                 dense_inverse(A, n, Ainv);
                 if (cl) cl->lock();
             }
+            // rows padded with zeros to a multiple of 8 elements, so every row starts 16-byte aligned
+            const int P = minv_pitch(n);
+            L->minv_pitch = P;
             if (t->cfg.mg_precision == 2) {   // FP16 storage of the coarsest level's inverse
-                std::vector<__half> h(Ainv.size());
-                for (size_t q = 0; q < Ainv.size(); q++) h[q] = __float2half((float)Ainv[q]);
+                std::vector<__half> h((size_t)n * P, __float2half(0.0f));
+                for (int r = 0; r < n; r++)
+                    for (int c = 0; c < n; c++) h[(size_t)r * P + c] = __float2half((float)Ainv[(size_t)r * n + c]);
                 HIPCHK(L->minv16.alloc(h.size()));
                 HIPCHK(hipMemcpy(L->minv16.p, h.data(), h.size() * sizeof(__half), hipMemcpyHostToDevice));
             } else {
-                HIPCHK(up(L->minv, Ainv));
+                std::vector<double> pad((size_t)n * P, 0.0);
+                for (int r = 0; r < n; r++) std::copy(&Ainv[(size_t)r * n], &Ainv[(size_t)r * n] + n, &pad[(size_t)r * P]);
+                HIPCHK(up(L->minv, pad));
             }
         }
         L->args.nxg = g.nxg; L->args.nyg = g.nyg; L->args.nzg = g.nzg; L->args.nxy = g.nxy;
@@ -266,6 +275,7 @@ int fill_bottom(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, Bott
     A.nlev = t->bot_top + 1;
     A.nu1 = t->cfg.nu1; A.nu2 = t->cfg.nu2;
     A.n0 = (int)mg[0]->g.n;
+    A.pitch0 = mg[0]->minv_pitch;
     A.minv = mg[0]->minv.p;
     for (int k = 0; k <= t->bot_top; k++) {
         MgLevel<T> &L = *mg[k];
@@ -401,11 +411,11 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     }
     if (k == 0) {
         if (L.minv16.p)
-            hipLaunchKernelGGL((k_mg_coarse<T, __half>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const __half *)L.minv16.p,
-                               (const T *)L.b.p, L.x.p, t->done());
+            hipLaunchKernelGGL((k_mg_coarse<T, __half>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, L.minv_pitch,
+                               (const __half *)L.minv16.p, (const T *)L.b.p, L.x.p, t->done());
         else
-            hipLaunchKernelGGL((k_mg_coarse<T, T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, (const T *)L.minv.p,
-                               (const T *)L.b.p, L.x.p, t->done());
+            hipLaunchKernelGGL((k_mg_coarse<T, T>), dim3(((int)L.g.n + 3) / 4), dim3(256), 0, s, (int)L.g.n, L.minv_pitch,
+                               (const T *)L.minv.p, (const T *)L.b.p, L.x.p, t->done());
         cur = L.x.p;
         return 0;
     }
