@@ -54,27 +54,45 @@ __device__ __forceinline__ void column_coefs(double d, double Ex, double Ey, dou
 // are issued before its first LDS store, so a block waits for HBM once, not once per row.
 // The two halves are also available separately (WindowRegs: load into registers, store to LDS) so that a
 // kernel can issue the loads of several windows before it waits for any of them.
+// FP32 windows of columns with an even NZ move two values per load (8 bytes): callers pass src at an even
+// element offset with an even pitch, so every pair is 8-byte aligned and stays inside one window row.
 template <typename T, int NZ, int MR, int MC, int NT>
 struct WindowRegs {
-    static constexpr int RL = MC * NZ, TOT = MR * RL, PER = (TOT + NT - 1) / NT;
-    T v[PER];
+    static constexpr int RL = MC * NZ, TOT = MR * RL;
+    static constexpr bool PAIRS = NZ % 2 == 0 && sizeof(T) == 4;
+    static constexpr int W = PAIRS ? 2 : 1, PER = (TOT / W + NT - 1) / NT;
+    T v[PER * W];
     __device__ __forceinline__ void load(const T *__restrict__ src, size_t pitch, int rows, int cols) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + q * NT;
+            const int e = ((int)threadIdx.x + q * NT) * W;
             const int r = e / RL, c = e - r * RL;
-            v[q] = (e < TOT && r < rows && c < cols * NZ) ? src[(size_t)r * pitch + c] : T(0);
+            const bool ok = e < TOT && r < rows && c < cols * NZ;
+            if constexpr (PAIRS) {
+                float2 w = make_float2(0.0f, 0.0f);
+                if (ok) w = *reinterpret_cast<const float2 *>(src + (size_t)r * pitch + c);
+                v[2 * q] = w.x;
+                v[2 * q + 1] = w.y;
+            } else {
+                v[q] = ok ? src[(size_t)r * pitch + c] : T(0);
+            }
         }
     }
     template <int NZP, int RS, typename U>
     __device__ __forceinline__ void store(U *lds, int rows, int cols) const {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + q * NT;
+            const int e = ((int)threadIdx.x + q * NT) * W;
             const int r = e / RL, c = e - r * RL;
             if (e < TOT && r < rows && c < cols * NZ) {
                 const int cc = c / NZ, k = c - cc * NZ;
-                lds[(r * RS + cc) * NZP + k] = (U)v[q];
+                const int a = (r * RS + cc) * NZP + k;
+                if constexpr (PAIRS) {
+                    lds[a] = (U)v[2 * q];
+                    lds[k + 1 < NZ ? a + 1 : a + NZP - NZ + 1] = (U)v[2 * q + 1];   // next level or next column
+                } else {
+                    lds[a] = (U)v[q];
+                }
             }
         }
     }
