@@ -97,6 +97,27 @@ struct WindowRegs {
         }
     }
 };
+// Copies one tile row of ncol columns from LDS (column c's NZ values at src + c * NZP) to dst (NZ values
+// per column, contiguous), converting to O.  For an even NZ two values go per store: dst is then at an even
+// element offset (a row of columns of an even depth), so every pair is aligned.
+template <int NZ, int NZP, int NT, typename T, typename O>
+__device__ __forceinline__ void store_row(O *__restrict__ dst, const T *src, int ncol) {
+    if constexpr (NZ % 2 == 0 && (sizeof(O) == 4 || sizeof(O) == 8)) {
+        using O2 = typename std::conditional<sizeof(O) == 4, float2, double2>::type;
+        for (int e = 2 * (int)threadIdx.x; e < ncol * NZ; e += 2 * NT) {
+            const int c = e / NZ, k = e - c * NZ, a = c * NZP + k;
+            O2 w;
+            w.x = (O)src[a];
+            w.y = (O)src[k + 1 < NZ ? a + 1 : a + NZP - NZ + 1];   // next level or next column
+            *reinterpret_cast<O2 *>(dst + e) = w;
+        }
+    } else {
+        for (int e = (int)threadIdx.x; e < ncol * NZ; e += NT) {
+            const int c = e / NZ, k = e - c * NZ;
+            dst[e] = (O)src[c * NZP + k];
+        }
+    }
+}
 template <typename T, int NZ, int NZP, int MR, int MC, int RS, int NT>
 __device__ __forceinline__ void stage_window(T *lds, const T *__restrict__ src, size_t pitch, int rows, int cols) {
     WindowRegs<T, NZ, MR, MC, NT> w;
@@ -2129,13 +2150,8 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     half(std::integral_constant<int, 1>{});
     using O = typename std::conditional<ZO, double, T>::type;
     O *const out = ZO ? (O *)uout : (O *)xout;
-    for (int r = 0; r < je - j0; r++) {
-        O *dst = out + ((size_t)(j0 + r) * nxg + i0) * NZ;
-        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
-            const int c = e / NZ, k = e - c * NZ;
-            dst[e] = (O)sX[((j0 + r - loy) * WX + (i0 + c - lox)) * NZP + k];
-        }
-    }
+    for (int r = 0; r < je - j0; r++)
+        store_row<NZ, NZP, NT>(out + ((size_t)(j0 + r) * nxg + i0) * NZ, sX + ((j0 + r - loy) * WX + (i0 - lox)) * NZP, ie - i0);
     // the ghost pass after the sweep for the ghost points this tile owns: wall columns beside its rows,
     // wall rows beside its columns, and the corners of the tiles at the corners of the domain
     {
@@ -2212,13 +2228,8 @@ __global__ __launch_bounds__(16 * TY) void k_mg_residual_t(LvArgs<T> L, const T 
     }
     __syncthreads();
     const int ie = cmin(i0 + tw, nxg), je = d3 ? cmin(j0 + TY, nyg) : 1;
-    for (int rr = 0; rr < je - j0; rr++) {
-        T *dst = r + ((size_t)(j0 + rr) * nxg + i0) * NZ;
-        for (int e = threadIdx.x; e < (ie - i0) * NZ; e += NT) {
-            const int c = e / NZ, k = e - c * NZ;
-            dst[e] = sX[((j0 + rr - loy) * rowstride + (i0 + c - lox)) * NZP + k];
-        }
-    }
+    for (int rr = 0; rr < je - j0; rr++)
+        store_row<NZ, NZP, NT>(r + ((size_t)(j0 + rr) * nxg + i0) * NZ, sX + ((j0 + rr - loy) * rowstride + (i0 - lox)) * NZP, ie - i0);
 }
 
 // Restriction (MGRestriction3DExplicit.f90): separable full weighting x -> y -> z on interior
