@@ -495,13 +495,25 @@ __global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__rest
     const int loy = cmax(cmin(j0 - R, nyg - 2 - 2 * A), 0);
     const int wx = cmin(WX, nxg - lox), wy = cmin(WY, nyg - loy);
     {
-        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT + NT - 1) / NT;
-        double v[PER];
+        // window values two at a time (16-byte loads) for an even NZ: rows of columns start at even
+        // offsets, so every pair is aligned and stays inside one window row
+        constexpr bool PAIRS = NZ % 2 == 0;
+        constexpr int W = PAIRS ? 2 : 1;
+        constexpr int RL = WX * NZ, TOT = WY * RL, PER = (TOT / W + NT - 1) / NT;
+        double v[PER * W];
         const double *src = phi + ((size_t)loy * nxg + lox) * NZ;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
-            v[q] = (e < TOT && r < wy && c < wx * NZ) ? src[(size_t)r * nxg * NZ + c] : 0.0;
+            const int e = ((int)threadIdx.x + q * NT) * W, r = e / RL, c = e - r * RL;
+            const bool ok = e < TOT && r < wy && c < wx * NZ;
+            if constexpr (PAIRS) {
+                double2 w = make_double2(0.0, 0.0);
+                if (ok) w = *reinterpret_cast<const double2 *>(src + (size_t)r * nxg * NZ + c);
+                v[2 * q] = w.x;
+                v[2 * q + 1] = w.y;
+            } else {
+                v[q] = ok ? src[(size_t)r * nxg * NZ + c] : 0.0;
+            }
         }
         for (int e = threadIdx.x; e < (TX + 2) * 2 * RA; e += NT) {
             const int r = e / (2 * RA), ii = i0 - 1 + r;
@@ -513,10 +525,16 @@ __global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__rest
         }
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = (int)threadIdx.x + q * NT, r = e / RL, c = e - r * RL;
+            const int e = ((int)threadIdx.x + q * NT) * W, r = e / RL, c = e - r * RL;
             if (e < TOT && r < wy && c < wx * NZ) {
                 const int cc = c / NZ, k = c - cc * NZ;
-                sP[k * PS + r * WX + cc] = v[q];
+                if constexpr (PAIRS) {
+                    sP[k * PS + r * WX + cc] = v[2 * q];
+                    if (k + 1 < NZ) sP[(k + 1) * PS + r * WX + cc] = v[2 * q + 1];
+                    else sP[r * WX + cc + 1] = v[2 * q + 1];   // level 0 of the next column
+                } else {
+                    sP[k * PS + r * WX + cc] = v[q];
+                }
             }
         }
     }
