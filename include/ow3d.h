@@ -102,7 +102,8 @@ int ow3d_stats(const ow3d_tank *t, long long *total, long long *solves, int *itm
 int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, double *V, double *W);
 
 /* Device-pointer form of ow3d_laplace_apply for callers that keep phi resident (stream: hipStream_t
- * or NULL for the handle's stream). */
+ * or NULL for the handle's stream).  d_phi must be 16-byte aligned (as hipMalloc returns), d_y 8-byte
+ * aligned; otherwise the call fails with an error. */
 int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, double *d_y, void *stream);
 /* The handle's HIP stream. */
 void *ow3d_stream(ow3d_tank *t);

@@ -322,6 +322,12 @@ extern "C" void *ow3d_stream(ow3d_tank *t) { return t ? (void *)t->s : nullptr; 
 
 extern "C" int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, double *d_y, void *stream) {
     if (!t) return -1;
+    if (!d_phi || !d_y) { err() = "null argument"; return -1; }
+    // the matvec kernels read phi's columns with 16-byte loads
+    if (reinterpret_cast<uintptr_t>(d_phi) % 16 != 0 || reinterpret_cast<uintptr_t>(d_y) % 8 != 0) {
+        err() = "d_phi must be 16-byte aligned and d_y 8-byte aligned";
+        return -1;
+    }
     hipStream_t s = stream ? (hipStream_t)stream : t->s;
     if (launch_matvec(t, d_phi, d_y, s)) { err() = "matvec dispatch failed"; return -1; }
     HIPCHK(hipGetLastError());

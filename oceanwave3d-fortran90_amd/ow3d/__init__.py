@@ -157,6 +157,13 @@ class Tank:
         _check(lib().ow3d_laplace_apply(self.h, _p(phi), _p(y)), "ow3d_laplace_apply")
         return y
 
+    def laplace_apply_device(self, d_phi, d_y, stream=None):
+        """y = A phi on device buffers given by address (e.g. a torch tensor's data_ptr(); d_phi 16-byte
+        aligned), enqueued on `stream` (a HIP stream handle) or the handle's own stream."""
+        _check(lib().ow3d_laplace_apply_device(self.h, ctypes.c_void_p(int(d_phi)), ctypes.c_void_p(int(d_y)),
+                                               ctypes.c_void_p(int(stream)) if stream else None),
+               "ow3d_laplace_apply_device")
+
     def precond_apply(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.shape3)
         u = np.empty(self.shape3)
