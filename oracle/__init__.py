@@ -79,6 +79,7 @@ def lib():
         L.owo_level_ghosts.restype = I
         L.owo_level_ghosts.argtypes = [P, I, ctypes.POINTER(I)]
         L.owo_gs_once.argtypes = [P, I, D, D]
+        L.owo_zebra.argtypes = [P, I, D, D, I]
         _lib = L
     return _lib
 
@@ -269,6 +270,14 @@ class OracleTank:
         b = np.ascontiguousarray(b, dtype=np.float64).ravel()
         x = np.array(x, dtype=np.float64, copy=True).ravel()
         lib().owo_gs_once(self.h, level, _p(b), _p(x))
+        return x
+
+    def zebra(self, level, b, x, colour):
+        """One zebra pass on a level: colour 0 / 1 relaxes the interior columns with (i + j) even / odd,
+        colour 2 is the wall-ghost pass."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        lib().owo_zebra(self.h, level, _p(b), _p(x), colour)
         return x
 
     def csr(self, level):

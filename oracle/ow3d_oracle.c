@@ -1326,6 +1326,11 @@ void owo_gs_once(owo_tank *t, int level, const double *b, double *x) {
     gs_csr(&t->lev[level], x, b);
     gs_special(&t->lev[level], x, b);
 }
+/* One colour of the zebra-line sweep, or (colour 2) the wall-ghost pass, on its own. */
+void owo_zebra(owo_tank *t, int level, const double *b, double *x, int colour) {
+    if (colour == 2) ghost_relax(&t->lev[level], x, b);
+    else zebra_colour(&t->lev[level], x, b, colour);
+}
 int owo_csr(owo_tank *t, int level, int *nrow, int *nnz, const int **rowptr, const int **col, const double **val) {
     level_t *L = &t->lev[level];
     if (!L->rp) return -1;

@@ -77,6 +77,9 @@ int  owo_level_grid(const owo_tank *t, int level, int *dims6, double *x, double 
                     double *hx, double *hy, double *hxx, double *hyy);
 int  owo_level_ghosts(const owo_tank *t, int level, int *out);
 void owo_gs_once(owo_tank *t, int level, const double *b, double *x);
+/* One pass of the zebra-line smoother on a level: colour 0 / 1 relaxes the interior sigma-columns with
+ * (i + j) even / odd, colour 2 is the wall-ghost relaxation pass. */
+void owo_zebra(owo_tank *t, int level, const double *b, double *x, int colour);
 
 #ifdef __cplusplus
 }

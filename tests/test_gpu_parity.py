@@ -81,34 +81,53 @@ def _hip():
     return hip
 
 
-def test_device_pointer_matvec():
-    """ow3d_laplace_apply_device on caller-owned device buffers gives the host entry point's bits, and a
-    d_phi that is not 16-byte aligned is refused with an error before any launch."""
+@pytest.mark.parametrize("nz", [9, 12])
+def test_device_pointer_matvec(nz):
+    """ow3d_laplace_apply_device on caller-owned device buffers gives the host entry point's bits, for an
+    even column depth (nz = 9: paired 16-byte window loads) and an odd one (nz = 12: single loads), on the
+    handle's stream and on a stream of the caller's (ordered after the handle's surface update); a d_phi
+    that is not 16-byte aligned or a d_y that is not 8-byte aligned is refused with an error before any
+    launch."""
     import ctypes
     ow3d = _ow3d()
     from ow3d import problems
     hip = _hip()
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
     bath = problems.gaussian_shoal(33, 33, 10.0, 10.0)
-    T = ow3d.Tank(dict(nx=33, ny=33, nz=9, lx=10.0, ly=10.0), **bath)
-    E, P = surface(33, 33, 10.0, seed=5)
-    T.set_surface(E, P)
+    T = ow3d.Tank(dict(nx=33, ny=33, nz=nz, lx=10.0, ly=10.0), **bath)
     phi = np.ascontiguousarray(np.random.default_rng(6).standard_normal(T.shape3))
     nbytes = phi.nbytes
-    d_phi, d_y = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(d_phi), nbytes) == 0 and hip.hipMalloc(ctypes.byref(d_y), nbytes) == 0
+    d_phi, d_y, st = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d_phi), nbytes) == 0 and hip.hipMalloc(ctypes.byref(d_y), nbytes + 16) == 0
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
     try:
         assert hip.hipMemcpy(d_phi, phi.ctypes.data, nbytes, 1) == 0   # host to device
-        for variant in (0, 3):   # column kernel; split-wave kernel (16-byte window loads)
+        for k, variant in enumerate((0, 3)):   # column kernel; split-wave kernel
             T.tune(variant, 0)
+            E, P = surface(33, 33, 10.0, seed=5 + k)
+            T.set_surface(E, P)
             y_host = T.laplace_apply(phi)
             T.laplace_apply_device(d_phi.value, d_y.value)
             assert hip.hipDeviceSynchronize() == 0   # the handle's stream does not order with the null stream
             y_dev = np.empty_like(phi)
             assert hip.hipMemcpy(y_dev.ctypes.data, d_y, nbytes, 2) == 0   # device to host
             assert np.array_equal(y_dev, y_host), variant
+            # a new surface, then the matvec on the caller's stream: it must see the new coefficients
+            E2, P2 = surface(33, 33, 10.0, seed=9 + k)
+            T.set_surface(E2, P2)
+            y_host2 = T.laplace_apply(phi)
+            T.laplace_apply_device(d_phi.value, d_y.value, st.value)
+            assert hip.hipStreamSynchronize(st) == 0
+            assert hip.hipMemcpy(y_dev.ctypes.data, d_y, nbytes, 2) == 0
+            assert np.array_equal(y_dev, y_host2), variant
         with pytest.raises(ow3d.OW3DError):
             T.laplace_apply_device(d_phi.value + 8, d_y.value)
+        with pytest.raises(ow3d.OW3DError):
+            T.laplace_apply_device(d_phi.value, d_y.value + 4)
     finally:
+        hip.hipStreamDestroy(st)
         hip.hipFree(d_phi)
         hip.hipFree(d_y)
     T.close()
@@ -280,6 +299,36 @@ def test_rk4_parity(oracle_mod, nx, ny, nz, shoal, sponge):
     assert rel(Eg, Eo) < 1e-10, rel(Eg, Eo)
     assert rel(Pg, Po) < 1e-10, rel(Pg, Po)
     assert rel(Wg, Wo) < 1e-9
+
+
+def test_rk4_matches_the_reference_faithful_oracle(oracle_mod):
+    """50 RK4 steps of a 33 x 33 x 9 shoal tank with sponges against the oracle in the reference's own
+    multigrid mode: CSR level matrices with the cross-derivative replication
+    (ConstructTableCrossDerivatives.f90:312-347) and lexicographic point Gauss-Seidel with the special-point
+    passes (GaussSeidel.f90:15-26, MultiGridSolver.f90:118-186).  The GPU smooths with zebra lines on the
+    unreplicated operator, so the preconditioners differ; with both solves driven to rtol 1e-13 in
+    GMRES(31) the trajectories must agree to 1e-10 in eta and Phi (the two oracle modes agree to 3e-12)."""
+    from ow3d import problems
+    L = 10.0
+    nx = ny = 33
+    T_o, T_g, _ = make_pair(oracle_mod, nx, ny, 9, shoal=True, reltol=1e-13, gmres_m=31, L=L,
+                            mg_mode=0, cd_quirk=1)
+    E, P = surface(nx, ny, L, seed=7, amp=0.01)
+    P = 0.01 * P
+    for (ax, i0, i1, j0, j1, gam) in problems.sponge_zones(nx, ny, L, L, 2.0):
+        T_o.add_relax_zone(ax, i0, i1, j0, j1, gam)
+        T_g.add_relax_zone(ax, i0, i1, j0, j1, gam)
+    T_o.set_state(E, P)
+    T_g.set_state(E, P)
+    dt, nsteps = 0.05, 50
+    for s in range(nsteps):
+        T_o.rk4_step(s * dt, dt)
+    T_g.rk4_steps(nsteps, 0.0, dt)
+    Eo, Po, Wo, _ = T_o.get_state()
+    Eg, Pg, Wg, _ = T_g.get_state()
+    assert rel(Eg, Eo) < 1e-10, rel(Eg, Eo)
+    assert rel(Pg, Po) < 1e-10, rel(Pg, Po)
+    assert rel(Wg, Wo) < 1e-9, rel(Wg, Wo)
 
 
 def test_fp16_coarse_storage(oracle_mod):
