@@ -49,12 +49,16 @@ def make_tank(ow3d, problems, mg_precision, solver=1, seed=0):
     return T, cfg, dt
 
 
-def run_steps(T, steps, dt, t0=0.0):
+def run_steps(T, steps, dt, t0=0.0, status=None):
+    """Per-step wall times of `steps` RK4 steps; status (a list) receives each step's return code
+    (1: a Laplace solve of that step missed its tolerance)."""
     lat = []
     for s in range(steps):
         a = time.perf_counter()
-        T.rk4_steps(1, t0 + s * dt, dt)
+        st = T.rk4_steps(1, t0 + s * dt, dt)
         lat.append(time.perf_counter() - a)
+        if status is not None:
+            status.append(st)
     return lat
 
 
@@ -154,11 +158,12 @@ def main():
         T, cfg, dt = make_tank(ow3d, problems, prec)
         run_steps(T, args.warmup, dt)
         T.set_state(*make_tank_state(problems))
-        lat = run_steps(T, args.steps, dt)
+        status = []
+        lat = run_steps(T, args.steps, dt, status=status)
         E, Pp, W, phi = T.get_state()
         tot, solves, itmin, itmax = T.stats()
         replays, captures, capture_ms = T.step_stats()
-        results[label] = dict(lat=lat, E=E, P=Pp, iters=(tot, solves, itmin, itmax),
+        results[label] = dict(lat=lat, E=E, P=Pp, iters=(tot, solves, itmin, itmax), unconverged_steps=sum(status),
                               graphs=dict(replays=replays, captures=captures, capture_ms=round(capture_ms, 2)))
         if label == "mixed":
             T.time_matvec(5)
@@ -194,9 +199,10 @@ def main():
                    "dof": dof, "solver": f"GMRES(23) rtol 1e-8 + V(1,1) zebra-line multigrid, {cfg['mg_maxgrids']} levels (mixed run)"},
         "fp64": {"value": dof * args.steps / t_f64, "ms_per_step": 1e3 * t_f64 / args.steps,
                  "p50_ms": 1e3 * float(np.percentile(f64["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(f64["lat"], 99)),
-                 "iters": f64["iters"], "step_graphs": f64["graphs"]},
+                 "iters": f64["iters"], "step_graphs": f64["graphs"], "unconverged_steps": f64["unconverged_steps"]},
         "mixed": {"p50_ms": 1e3 * float(np.percentile(mix["lat"], 50)), "p99_ms": 1e3 * float(np.percentile(mix["lat"], 99)),
-                  "iters": mix["iters"], "step_graphs": mix["graphs"], "rel_err_eta_vs_fp64": err_eta,
+                  "iters": mix["iters"], "step_graphs": mix["graphs"], "unconverged_steps": mix["unconverged_steps"],
+                  "rel_err_eta_vs_fp64": err_eta,
                   "rel_err_phi_vs_fp64": err_phi},
         "speedup_mixed_vs_fp64": t_f64 / t_mix,
         "roofline": {"bound": "hbm", "kernel": f"{mv_kernel} (high-order sigma-Laplace matvec)", "achieved": achieved,

@@ -103,7 +103,10 @@ int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, double *V, doubl
 
 /* Device-pointer form of ow3d_laplace_apply for callers that keep phi resident (stream: hipStream_t
  * or NULL for the handle's stream).  d_phi must be 16-byte aligned (as hipMalloc returns), d_y 8-byte
- * aligned; otherwise the call fails with an error. */
+ * aligned; otherwise the call fails with an error.  The operator's sigma coefficients are rewritten on the
+ * handle's stream by surface updates and RK4 stages: on another stream the matvec is ordered after the
+ * handle's work enqueued so far, and the handle's later work after the matvec.  Fails while asynchronous
+ * steps are in flight. */
 int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, double *d_y, void *stream);
 /* The handle's HIP stream. */
 void *ow3d_stream(ow3d_tank *t);
@@ -123,6 +126,10 @@ int ow3d_batch_rk4_steps(ow3d_tank *const *tanks, int ntanks, int nsteps, const 
 int ow3d_rk4_steps_async(ow3d_tank *t, int nsteps, double dt);
 int ow3d_query(ow3d_tank *t);
 int ow3d_wait(ow3d_tank *t);
+/* 1 if a Laplace solve of the steps enqueued by the last ow3d_rk4_steps / ow3d_rk4_steps_async /
+ * ow3d_batch_rk4_steps call missed its tolerance, 0 if all converged; an error while steps are in flight.
+ * (The reference ignores non-convergence, iterative_solution.f90:115-134.) */
+int ow3d_steps_status(ow3d_tank *t);
 /* Each stage's solve is enqueued with the iteration slots recent steps used plus a margin; a step in
  * which a solve used all its slots without stopping is rolled back and replayed with the whole budget,
  * so results equal those of whole-budget steps.  Stages are captured per (stage, slots).

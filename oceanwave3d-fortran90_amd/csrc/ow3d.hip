@@ -29,6 +29,11 @@ std::atomic<long long> &ow3d_host::device_bytes() {
     return b;
 }
 
+long long &ow3d_host::thread_bytes() {
+    static thread_local long long b = 0;
+    return b;
+}
+
 namespace {
 
 // ---------------------------------------------------------------------------------------------
@@ -233,12 +238,13 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (hxx) std::copy(hxx, hxx + nxy, g.hxx.begin());
     if (hyy) std::copy(hyy, hyy + nxy, g.hyy.begin());
     grid_stencils(g);
-    const long long bytes0 = device_bytes();
+    const long long bytes0 = thread_bytes();
     // non-blocking: set-up copies of a new handle (null stream) do not wait for other handles' work
     HIPCHK(hipStreamCreateWithFlags(&t->s, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&t->ev0));
     HIPCHK(hipEventCreate(&t->ev1));
     HIPCHK(hipEventCreateWithFlags(&t->ev_done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&t->ev_order, hipEventDisableTiming));
     hipStream_t s = t->s;
     auto upd = [&](DevBuf<double> &d, const std::vector<double> &v) -> hipError_t {
         hipError_t e = d.alloc(v.size());
@@ -296,7 +302,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     for (int &v : t->slots) v = full_slots(t.get());
     HIPCHK(hipStreamSynchronize(nullptr));
     HIPCHK(hipStreamSynchronize(t->s));
-    t->bytes = device_bytes() - bytes0;
+    t->bytes = thread_bytes() - bytes0;
     (void)s;
     *out = t.release();
     return 0;
@@ -328,9 +334,19 @@ extern "C" int ow3d_laplace_apply_device(ow3d_tank *t, const double *d_phi, doub
         err() = "d_phi must be 16-byte aligned and d_y 8-byte aligned";
         return -1;
     }
+    // the operator's sigma coefficients are rewritten on the handle's stream by every surface update
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
     hipStream_t s = stream ? (hipStream_t)stream : t->s;
+    if (s != t->s) {   // the caller's stream waits for the handle's work, and the handle's for this matvec
+        HIPCHK(hipEventRecord(t->ev_order, t->s));
+        HIPCHK(hipStreamWaitEvent(s, t->ev_order, 0));
+    }
     if (launch_matvec(t, d_phi, d_y, s)) { err() = "matvec dispatch failed"; return -1; }
     HIPCHK(hipGetLastError());
+    if (s != t->s) {
+        HIPCHK(hipEventRecord(t->ev_order, s));
+        HIPCHK(hipStreamWaitEvent(t->s, t->ev_order, 0));
+    }
     return 0;
 }
 
@@ -459,10 +475,12 @@ extern "C" int ow3d_add_relax_zone(ow3d_tank *t, int axis, int i0, int i1, int j
     int len = axis == 0 ? i1 - i0 + 1 : j1 - j0 + 1;
     t->gam.insert(t->gam.end(), gam, gam + len);
     t->zones.push_back(z);
+    const long long b0 = thread_bytes();
     HIPCHK(t->dzones.alloc(t->zones.size()));
     HIPCHK(hipMemcpy(t->dzones.p, t->zones.data(), t->zones.size() * sizeof(Zone), hipMemcpyHostToDevice));
     HIPCHK(t->dgam.alloc(t->gam.size()));
     HIPCHK(hipMemcpy(t->dgam.p, t->gam.data(), t->gam.size() * 8, hipMemcpyHostToDevice));
+    t->bytes += thread_bytes() - b0;
     return 0;
 }
 
@@ -685,6 +703,13 @@ extern "C" int ow3d_wait(ow3d_tank *t) {
         if (r < 0) return r;
         if (r == 1) break;
     }
+    return steps_status(t);
+}
+
+extern "C" int ow3d_steps_status(ow3d_tank *t) {
+    if (!t) { err() = "null handle"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    HIPCHK(hipStreamSynchronize(t->s));
     return steps_status(t);
 }
 

@@ -41,6 +41,9 @@ constexpr int RED_BLOCKS = 2048;
 
 // bytes of device memory held by the library's buffers (all handles of the process)
 std::atomic<long long> &device_bytes();
+// bytes allocated minus bytes freed by the library's buffers on this host thread: a handle's own
+// footprint is the change over its creation, whatever other threads allocate or free meanwhile
+long long &thread_bytes();
 
 // Process-wide lock held while a stream is captured into a graph and while a handle is created,
 // destroyed or has state copied in or out (allocation, frees and blocking copies on other threads must
@@ -59,7 +62,7 @@ struct DevBuf {
     size_t n = 0;
     ~DevBuf() { release(); }
     void release() {
-        if (p) { (void)hipFree(p); device_bytes() -= (long long)bytes(); }
+        if (p) { (void)hipFree(p); device_bytes() -= (long long)bytes(); thread_bytes() -= (long long)bytes(); }
         p = nullptr; n = 0;
     }
     size_t bytes() const { return (n ? n : 1) * sizeof(T); }
@@ -67,7 +70,7 @@ struct DevBuf {
         release();
         n = count;
         hipError_t e = hipMalloc((void **)&p, bytes());
-        if (e == hipSuccess) device_bytes() += (long long)bytes();
+        if (e == hipSuccess) { device_bytes() += (long long)bytes(); thread_bytes() += (long long)bytes(); }
         else { p = nullptr; n = 0; }
         return e;
     }
@@ -136,6 +139,7 @@ struct ow3d_tank {
     long tstep = 0;
     long long bytes = 0;                // device memory of this handle
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    hipEvent_t ev_order = nullptr;      // orders a caller's stream with the handle's (device-pointer entry points)
     bool use_graph = true;
     bool capturing = false;             // a whole RK4 step is being captured (no nested graph launches)
     hipGraphExec_t pg_exec = nullptr;   // captured preconditioner (V cycles)
@@ -167,6 +171,7 @@ struct ow3d_tank {
         if (report) (void)hipHostFree(report);
         if (pg_exec) (void)hipGraphExecDestroy(pg_exec);
         if (ev_done) (void)hipEventDestroy(ev_done);
+        if (ev_order) (void)hipEventDestroy(ev_order);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (s) (void)hipStreamDestroy(s);

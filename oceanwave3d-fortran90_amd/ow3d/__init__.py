@@ -53,7 +53,7 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
            "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
-           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics"]
+           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics", "ow3d_steps_status"]
 
 
 class OW3DError(RuntimeError):
@@ -95,6 +95,7 @@ def lib():
         L.ow3d_rk4_steps_async.argtypes = [P, I, ctypes.c_double]
         L.ow3d_query.argtypes = [P]
         L.ow3d_wait.argtypes = [P]
+        L.ow3d_steps_status.argtypes = [P]
         L.ow3d_device_bytes.restype = ctypes.c_longlong
         L.ow3d_device_bytes.argtypes = [P]
         L.ow3d_tune.argtypes = [P, I, I]
@@ -240,6 +241,11 @@ class Tank:
     def wait(self):
         """Blocks until the enqueued steps are done; 1 if a solve missed its tolerance."""
         return _check(lib().ow3d_wait(self.h), "ow3d_wait")
+
+    def steps_status(self):
+        """1 if a solve of the steps enqueued by the last rk4_steps / rk4_steps_async / batch call missed
+        its tolerance, else 0 (the steps must have completed)."""
+        return _check(lib().ow3d_steps_status(self.h), "ow3d_steps_status")
 
     def device_bytes(self):
         return int(lib().ow3d_device_bytes(self.h))

@@ -10,8 +10,11 @@ between steps (continuous batching, no barrier between tanks).  Tanks are built 
 
 Reported: aggregate DOF-steps/s (DOF = Nx*Ny*(Nz+1) per tank, as OceanWave3D.f90:29), p50/p99 of the
 per-step latency (from the moment a step is enqueued to the moment the loop sees it settled), queueing
-delay, admissions and completions.  Each request's result does not depend on what else runs: tanks
-share nothing on the device, so a request's final state is bit-identical alone and in the mix.
+delay, admissions and completions, and how many completed requests had a Laplace solve that missed its
+tolerance (the reference ignores this, iterative_solution.f90:115-134).  A request whose tank can never
+fit the HBM budget is rejected on arrival instead of blocking the queue.  Each request's result does not
+depend on what else runs: tanks share nothing on the device, so a request's final state is bit-identical
+alone and in the mix.
 
     python -m ow3d.serve --rate 20 --duration 20 --max-active 32 --precision both
 """
@@ -87,13 +90,14 @@ def run_alone(req):
 
 
 class _Active:
-    __slots__ = ("req", "tank", "dt", "done", "submitted", "admitted")
+    __slots__ = ("req", "tank", "dt", "done", "submitted", "admitted", "unconverged")
 
     def __init__(self, req, tank, dt, now):
         self.req, self.tank, self.dt = req, tank, dt
         self.done = 0
         self.submitted = now
         self.admitted = now
+        self.unconverged = 0   # steps in which a solve missed its tolerance
 
 
 class Server:
@@ -136,12 +140,17 @@ class Server:
         in_build = 0
         lat, qdelay, results = [], [], {}
         steps_done = dof_steps = completed = 0
+        rejected = []                      # request ids larger than the whole budget
+        unconverged_requests = unconverged_steps = 0
         t0 = time.perf_counter()
         try:
             while waiting or active or in_build:
                 now = time.perf_counter() - t0
                 if time_limit is not None and now > time_limit:
                     break
+                # a request that cannot fit even an empty budget is rejected when it arrives
+                while waiting and waiting[0].arrival <= now and waiting[0].bytes > self.hbm_budget:
+                    rejected.append(waiting.popleft().rid)
                 # admission: arrived, a free slot, memory within the budget
                 while (waiting and waiting[0].arrival <= now and len(active) + in_build < self.max_active
                        and reserved + waiting[0].bytes <= self.hbm_budget):
@@ -150,8 +159,6 @@ class Server:
                     in_build += 1
                     qdelay.append(now - req.arrival)
                     want.put(req)
-                    if not waiting or waiting[0].bytes > self.hbm_budget:
-                        break
                 while True:
                     try:
                         req, tk, e = built.get_nowait()
@@ -173,6 +180,7 @@ class Server:
                     t = time.perf_counter() - t0
                     lat.append(t - a.submitted)
                     a.done += 1
+                    a.unconverged += a.tank.steps_status()
                     steps_done += 1
                     dof_steps += a.req.dof
                     if a.done < a.req.nsteps:
@@ -184,6 +192,8 @@ class Server:
                     a.tank.close()
                     reserved -= a.req.bytes
                     completed += 1
+                    unconverged_steps += a.unconverged
+                    unconverged_requests += a.unconverged > 0
                     del active[rid]
                 if not progressed:
                     time.sleep(self.poll_s)
@@ -204,6 +214,9 @@ class Server:
             "requests": len(requests),
             "admitted": len(qdelay),
             "completed": completed,
+            "rejected": rejected,
+            "completed_with_unconverged_solves": unconverged_requests,
+            "unconverged_steps_of_completed": unconverged_steps,
             "steps": steps_done,
             "dof_steps_per_s": dof_steps / wall if wall > 0 else 0.0,
             "step_latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),

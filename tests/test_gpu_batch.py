@@ -118,6 +118,31 @@ def test_estimated_device_bytes_match_the_allocation(prec):
         T.close()
 
 
+def test_footprint_of_a_handle_created_while_another_is_destroyed():
+    """A handle's device_bytes counts its own buffers only: destroying another handle on another thread
+    while it is being created (creation releases the capture lock during its host-side coarse inverse)
+    must not change it."""
+    import threading
+    from ow3d import problems
+    spec = problems.batch_tank(81, sizes=(129,), nzs=(9,), orders=(6,), bottom="shoal")
+    ref, _ = make(spec)
+    want = ref.device_bytes()
+    ref.close()
+    others = [make(s)[0] for s in specs([82, 83, 84], sizes=(65,))]
+    out = {}
+
+    def build():
+        out["T"] = make(spec)[0]
+
+    th = threading.Thread(target=build)
+    th.start()
+    for T in others:
+        T.close()
+    th.join()
+    assert out["T"].device_bytes() == want
+    out["T"].close()
+
+
 def test_even_nz_multigrid_matches_the_oracle(oracle_mod):
     """Nz = 12: the multigrid keeps the even column and halves x and y down to 3 points (17, 9, 5, 3);
     the oracle follows the same rule and must agree with the GPU over RK4 steps."""
@@ -179,6 +204,19 @@ def test_request_loop_results_equal_each_request_alone():
     assert out["queue_delay_s"]["p99"] > 0.0          # some requests waited for a slot
     for r in reqs:
         assert same(out["results"][r.rid], serve.run_alone(r)), r.rid
+
+
+def test_request_loop_reports_unconverged_solves():
+    """Solves limited to GMRES(6) at rtol 1e-14 cannot reach their tolerance: the loop must count every
+    such step per request, as ow3d_steps_status reports it."""
+    from ow3d import serve
+    reqs = serve.poisson_requests(3, rate=1000.0, seed=9, mg_precision=1, steps=(2, 3), sizes=(33,))
+    for r in reqs:
+        r.cfg.update(gmres_m=6, reltol=1e-14)
+    out = serve.Server(hbm_budget=50e9, max_active=3).run(reqs)
+    assert out["completed"] == 3
+    assert out["completed_with_unconverged_solves"] == 3
+    assert out["unconverged_steps_of_completed"] == out["steps"]
 
 
 def test_request_loop_respects_the_memory_budget():

@@ -83,3 +83,13 @@ def test_autotuner_candidates_and_cache(tmp_path):
     back = autotune.load(path)
     assert autotune.tuned(dict(nx=65, ny=65, nz=9, alpha=2), back)["matvec_variant"] == 1
     assert autotune.tuned(dict(nx=65, ny=65, nz=9, alpha=3), back) is None
+
+
+def test_requests_larger_than_the_budget_are_rejected_on_arrival():
+    """A request whose tank cannot fit even the whole HBM budget is rejected when it arrives rather than
+    blocking the queue: with every request too large the loop returns at once, having built nothing."""
+    from ow3d import serve
+    reqs = serve.poisson_requests(3, rate=1000.0, seed=2, mg_precision=1, steps=(2, 3), sizes=(65,))
+    out = serve.Server(hbm_budget=min(r.bytes for r in reqs) / 2, max_active=4).run(reqs, time_limit=None)
+    assert sorted(out["rejected"]) == [r.rid for r in reqs]
+    assert out["admitted"] == 0 and out["completed"] == 0 and out["steps"] == 0
