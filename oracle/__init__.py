@@ -328,6 +328,7 @@ def ref_lib():
         R.ref_gs.argtypes = [I, I, IP, IP, D, D, D, I, IP]
         R.ref_transfer.argtypes = [I, I, I, I, D, D, D, I, I, I, D, D, D, D, D]
         R.ref_store_ep.argtypes = [D, D, I]
+        R.ref_jonswap.argtypes = [I, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
         _ref = R
     return _ref
 
@@ -433,3 +434,11 @@ def ref_transfer(direction, fine_dims, fine_xyz, coarse_dims, coarse_xyz, fine, 
     coarse = np.array(coarse, dtype=np.float64, copy=True)
     ref_lib().ref_transfer(direction, *fine_dims, *[_p(a) for a in f], *coarse_dims, *[_p(a) for a in c], _p(fine), _p(coarse))
     return coarse if direction == 0 else fine
+
+
+def ref_jonswap(freq_hz, hs, tp, gamma):
+    """JONSWAP_spectrum (utilities/JONSWAP_spectrum.f90) at the given frequencies (Hz)."""
+    f = np.ascontiguousarray(freq_hz, dtype=np.float64)
+    out = np.zeros_like(f)
+    ref_lib().ref_jonswap(len(f), _p(f), float(hs), float(tp), float(gamma), _p(out))
+    return out

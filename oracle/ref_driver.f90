@@ -301,4 +301,19 @@ CONTAINS
     CALL StoreData(nxg, nyg, E(1:nxg*nyg), P(1:nxg*nyg), FineGrid, INT(nr), 1)
   END SUBROUTINE ref_store_ep
 
+  ! JONSWAP_spectrum(FREQ, Hs, Tp, gamma) (utilities/JONSWAP_spectrum.f90) at n frequencies in Hz.
+  SUBROUTINE ref_jonswap(n, freq, hs, tp, gam, out) BIND(C, name="ref_jonswap")
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), VALUE :: hs, tp, gam
+    REAL(C_DOUBLE), DIMENSION(*) :: freq, out
+    REAL(KIND=long), EXTERNAL :: JONSWAP_spectrum
+    REAL(KIND=long) :: f, h, t, gg
+    INTEGER :: i
+    h = hs; t = tp; gg = gam
+    DO i = 1, n
+       f = freq(i)
+       out(i) = JONSWAP_spectrum(f, h, t, gg)
+    END DO
+  END SUBROUTINE ref_jonswap
+
 END MODULE ow3d_ref_driver

@@ -11,6 +11,7 @@ outputs for them:
   W                                  VerticalFreeSurfaceVelocity.f90
   rhsE, rhsP                         rhsFreeSurface3D.f90
   restriction / prolongation         MGRestriction3DExplicit.f90, MGProlongation3DExplicit.f90
+  JONSWAP spectrum (jonswap_ref.npz) JONSWAP_spectrum.f90:1-30
 
 Usage: python tests/golden/make_golden.py   (writes tests/golden/ref_<case>.npz)
 """
@@ -84,6 +85,13 @@ def main():
         out.update({f"rf{fine}": rf, f"rc{fine}": rc, f"ec{fine}": ec, f"ef{fine}": ef})
     np.savez_compressed(os.path.join(HERE, "ref_transfer17x17x9.npz"), **out)
     print("wrote transfer17x17x9")
+    # JONSWAP spectrum: two peak periods and peak enhancements over 0.05 .. 4 times the peak frequency
+    out = {}
+    for q, (hs, tp, gam) in enumerate(((0.02, 1.72, 3.3), (0.5, 6.0, 1.0))):
+        f = np.linspace(0.05, 4.0, 400) / tp
+        out.update({f"f{q}": f, f"S{q}": oracle.ref_jonswap(f, hs, tp, gam), f"par{q}": np.array([hs, tp, gam])})
+    np.savez_compressed(os.path.join(HERE, "jonswap_ref.npz"), **out)
+    print("wrote jonswap_ref")
 
 
 if __name__ == "__main__":
