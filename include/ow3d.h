@@ -68,6 +68,15 @@ int ow3d_precond_apply(ow3d_tank *t, const double *r, double *u);
  * residual norm (estimate for GMRES). */
 int ow3d_solve(ow3d_tank *t, const double *rhs, double *sol, int *iters, double *res);
 
+/* ow3d_solve that allocates no device memory (no counterpart in the reference; for callers that cannot
+ * allocate: tight request loops, harnesses that forbid hipMalloc).  Same arguments, results and return
+ * codes as ow3d_solve, bit for bit.  Its workspace -- reduction partial sums, the staged rhs and solution,
+ * the residual and correction vectors and the Krylov basis, (GMRESmaxiterations + 5) vectors of the grid
+ * -- lives in one static 64 MiB device array per GPU, held until the process ends; a tank whose workspace
+ * does not fit (e.g. 257 x 257 x 9 at GMRESmaxiterations 23) is refused with an error.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (several host threads) race. */
+int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, int *iters, double *res);
+
 /* Replaces DifferentiationsFreeSurfacePlane(Wavefield,...) (timeintegration/DifferentiationsFreeSurfacePlane.f90:1-32)
  * followed by DetermineTransformationConstantsArray (functions/DetermineTransformationConstantsArray.f90):
  * updates the wall ghost values of E and P in place and sets the operator's sigma coefficients.

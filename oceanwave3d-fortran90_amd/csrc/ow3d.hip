@@ -78,9 +78,16 @@ int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, 
     return -1;
 }
 
+// Workspace of one Laplace solve: the Krylov basis V (m + 1 vectors of n), the residual w and correction z
+// vectors, the partial sums of the reductions (RED_BLOCKS x MAXK) and their totals hv (MAXK).
+struct SolveWork {
+    double *V, *w, *z, *part, *hv;
+};
+SolveWork handle_work(ow3d_tank *t) { return {t->V.p, t->w.p, t->z.p, t->part.p, t->hv.p}; }
+
 // hv[0..k-1] = sum over the partial sums of the last reduction
-void reduce(ow3d_tank *t, int k, int nb, const int *skip, hipStream_t s) {
-    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)t->part.p, t->hv.p, skip);
+void reduce(const SolveWork &W, int k, int nb, const int *skip, hipStream_t s) {
+    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)W.part, W.hv, skip);
 }
 
 // Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres): classical
@@ -94,10 +101,10 @@ int full_slots(const ow3d_tank *t) {
 }
 
 // slots: Arnoldi steps enqueued (at most full_slots); stage: RK4 stage (0..3) the slot use is kept for.
-int gmres(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, int stage) {
+int gmres(ow3d_tank *t, const SolveWork &W, const double *b, double *x, hipStream_t s, int slots, int stage) {
     const size_t n = t->g.n;
     const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
-    double *V = t->V.p, *w = t->w.p;
+    double *V = W.V, *w = W.w;
     KrylovState *S = t->ks.p;
     const int *done = t->done(), *skip2 = t->skip2();
     const int nbl = grid_blocks(n), nb = red_blocks(n);
@@ -105,22 +112,22 @@ int gmres(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, in
     if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
     hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
     if (precond(t, w, V, s)) return -1;
-    if (launch_mdot(1, false, nb, n, V, V, t->part.p, done, s)) return -1;
-    reduce(t, 1, nb, done, s);
-    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, t->cfg.reltol, t->cfg.abstol);
+    if (launch_mdot(1, false, nb, n, V, V, W.part, done, s)) return -1;
+    reduce(W, 1, nb, done, s);
+    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, t->cfg.reltol, t->cfg.abstol);
     hipLaunchKernelGGL(k_scale_state, dim3(nbl), dim3(256), 0, s, n, (const KrylovState *)S, V);
     for (int k = 1; k <= slots; k++) {
         double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
         if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
         if (precond(t, w, vn, s)) return -1;
-        if (launch_mdot(k, true, nb, n, V, vn, t->part.p, done, s)) return -1;
-        reduce(t, k + 1, nb, done, s);
-        hipLaunchKernelGGL(k_gmres_pass1, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, k, m);
-        if (launch_gs_update_dot(k, nb, n, V, t->hv.p, vn, t->part.p, skip2, s)) return -1;
-        reduce(t, k + 1, nb, skip2, s);
-        hipLaunchKernelGGL(k_gmres_pass2, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, k, m);
+        if (launch_mdot(k, true, nb, n, V, vn, W.part, done, s)) return -1;
+        reduce(W, k + 1, nb, done, s);
+        hipLaunchKernelGGL(k_gmres_pass1, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, k, m);
+        if (launch_gs_update_dot(k, nb, n, V, W.hv, vn, W.part, skip2, s)) return -1;
+        reduce(W, k + 1, nb, skip2, s);
+        hipLaunchKernelGGL(k_gmres_pass2, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, k, m);
         // vn = (vn - V h) / hn with the projections of the last pass
-        if (launch_gs_update(k, nb, n, V, t->hv.p, &S->scale, vn, done, s)) return -1;
+        if (launch_gs_update(k, nb, n, V, W.hv, &S->scale, vn, done, s)) return -1;
     }
     hipLaunchKernelGGL(k_gmres_finish, dim3(1), dim3(64), 0, s, S, slots, full_slots(t), stage);
     hipLaunchKernelGGL(k_basis_combine, dim3(nbl), dim3(256), 0, s, n, (const double *)V, (const KrylovState *)S, x);
@@ -128,10 +135,10 @@ int gmres(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, in
 }
 
 // Defect correction (dc.f:1-151), at most m iterations (slots of them enqueued).
-int dcorr(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, int stage) {
+int dcorr(ow3d_tank *t, const SolveWork &W, const double *b, double *x, hipStream_t s, int slots, int stage) {
     const size_t n = t->g.n;
     const int nbl = grid_blocks(n), nb = red_blocks(n);
-    double *w = t->w.p, *z = t->z.p;
+    double *w = W.w, *z = W.z;
     KrylovState *S = t->ks.p;
     const int *done = t->done();
     hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, s, S);
@@ -139,20 +146,24 @@ int dcorr(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots, in
         if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
         hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
         if (precond(t, w, z, s)) return -1;
-        if (launch_mdot(1, false, nb, n, z, z, t->part.p, done, s)) return -1;
-        reduce(t, 1, nb, done, s);
-        hipLaunchKernelGGL(k_dc_check, dim3(1), dim3(64), 0, s, S, (const double *)t->hv.p, it, t->cfg.reltol, t->cfg.abstol);
+        if (launch_mdot(1, false, nb, n, z, z, W.part, done, s)) return -1;
+        reduce(W, 1, nb, done, s);
+        hipLaunchKernelGGL(k_dc_check, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, it, t->cfg.reltol, t->cfg.abstol);
         hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(256), 0, s, n, 1.0, (const double *)z, 1.0, x, done);
     }
     hipLaunchKernelGGL(k_dc_finish, dim3(1), dim3(64), 0, s, S, slots, full_slots(t), stage);
     return 0;
 }
 
-// Enqueues one Laplace solve of A x = b (x: initial guess in, solution out); slots < 0: whole budget.
-int solve_dev(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots = -1, int stage = 0) {
+// Enqueues one Laplace solve of A x = b (x: initial guess in, solution out) in workspace W (the handle's
+// unless given); slots < 0: whole budget.
+int solve_dev(ow3d_tank *t, const double *b, double *x, hipStream_t s, int slots = -1, int stage = 0,
+              const SolveWork *W = nullptr) {
     const int f = full_slots(t);
     if (slots < 0 || slots > f) slots = f;
-    return t->cfg.solver == 0 ? dcorr(t, b, x, s, slots, stage) : gmres(t, b, x, s, slots, stage);
+    const SolveWork hw = handle_work(t);
+    if (!W) W = &hw;
+    return t->cfg.solver == 0 ? dcorr(t, *W, b, x, s, slots, stage) : gmres(t, *W, b, x, s, slots, stage);
 }
 
 PlaneArgs plane_args(ow3d_tank *t, double *E, double *P) {
@@ -190,7 +201,43 @@ __global__ void k_extrap3(size_t n, const double *__restrict__ Lnew, const doubl
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         phi[q] = 2.66667 * phi[q] - 2.0 * Lnew[q] + 0.333333 * Llast[q];
 }
+__global__ void k_zero(size_t n, double *__restrict__ v) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) v[q] = 0.0;
+}
 }  // namespace ow3d
+
+// Workspace of ow3d_solve_noalloc: a static device array, one copy per GPU the library runs on, held
+// until the process ends.  Host code reaches it only through the address hipGetSymbolAddress gives for the
+// current device (the array's name is a host-side stand-in, not a device address).
+namespace {
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+constexpr size_t SCRATCH_DOUBLES = SCRATCH_BYTES / sizeof(double);
+constexpr int SCRATCH_MAX_DEVICES = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The scratch array's address on the current device (looked up once per device).
+int scratch_ptr(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) { err() = "device index beyond the scratch table"; return -1; }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+// Vectors in the scratch array start on 256-byte boundaries.
+inline size_t pad32(size_t n) { return (n + 31) & ~size_t(31); }
+}  // namespace
 
 // ---------------------------------------------------------------------------------------------
 // C ABI
@@ -397,6 +444,42 @@ extern "C" int ow3d_solve(ow3d_tank *t, const double *rhs, double *sol, int *ite
     KrylovState k{};
     HIPCHK(hipMemcpyAsync(&k, t->ks.p, sizeof(k), hipMemcpyDeviceToHost, t->s));
     HIPCHK(hipStreamSynchronize(t->s));
+    if (iters) *iters = k.matvecs;
+    if (res) *res = k.est;
+    return k.status;
+}
+
+// ow3d_solve with its workspace in the static scratch array: the reduction partial sums and totals, the
+// staged right-hand side and solution, the residual and correction vectors and the Krylov basis, laid out
+// in that order.  The V cycle is enqueued inline (the handle's captured preconditioner graph would be
+// instantiated on first use).  No device memory is allocated or freed.
+extern "C" int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, int *iters, double *res) {
+    if (!t || !rhs || !sol) { err() = "null argument"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    const size_t n = t->g.n, np = pad32(n);
+    const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
+    const size_t head = pad32((size_t)RED_BLOCKS * MAXK) + pad32(MAXK);
+    const size_t need = head + (size_t)(4 + m + 1) * np;
+    if (need > SCRATCH_DOUBLES) {
+        err() = "tank too large for the no-allocation solve: its workspace needs " + std::to_string(need * 8 >> 20) +
+                " MiB, the static array holds " + std::to_string(SCRATCH_BYTES >> 20) + " MiB";
+        return -1;
+    }
+    double *base = nullptr;
+    if (scratch_ptr(&base)) return -1;
+    double *b = base + head, *x = b + np;
+    SolveWork W{x + 3 * np, x + np, x + 2 * np, base, base + pad32((size_t)RED_BLOCKS * MAXK)};
+    hipStream_t s = t->s;
+    // the array keeps the last call's contents: the basis starts from zeros, as the handle's does
+    hipLaunchKernelGGL(k_zero, dim3(grid_blocks((size_t)(m + 1) * np)), dim3(256), 0, s, (size_t)(m + 1) * np, W.V);
+    HIPCHK(hipMemcpyAsync(b, rhs, n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(x, sol, n * 8, hipMemcpyHostToDevice, s));
+    if (solve_dev(t, b, x, s, -1, 0, &W)) return -1;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sol, x, n * 8, hipMemcpyDeviceToHost, s));
+    KrylovState k{};
+    HIPCHK(hipMemcpyAsync(&k, t->ks.p, sizeof(k), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     if (iters) *iters = k.matvecs;
     if (res) *res = k.est;
     return k.status;

@@ -53,7 +53,8 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_rhs_free_surface", "ow3d_add_relax_zone", "ow3d_set_state", "ow3d_get_state",
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
            "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
-           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics", "ow3d_steps_status"]
+           "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics", "ow3d_steps_status",
+           "ow3d_solve_noalloc"]
 
 
 class OW3DError(RuntimeError):
@@ -79,6 +80,7 @@ def lib():
         L.ow3d_laplace_apply.argtypes = [P, D, D]
         L.ow3d_precond_apply.argtypes = [P, D, D]
         L.ow3d_solve.argtypes = [P, D, D, IP, D]
+        L.ow3d_solve_noalloc.argtypes = [P, D, D, IP, D]
         L.ow3d_set_surface.argtypes = [P, D, D, D, D, D, D, D, D]
         L.ow3d_vertical_velocity.argtypes = [P, D, D]
         L.ow3d_rhs_free_surface.argtypes = [P, D, D, D, D]
@@ -177,6 +179,16 @@ class Tank:
         it = ctypes.c_int(0)
         res = ctypes.c_double(0.0)
         st = _check(lib().ow3d_solve(self.h, _p(rhs), _p(x), ctypes.byref(it), ctypes.pointer(res)), "ow3d_solve")
+        return x, st, it.value, res.value
+
+    def solve_noalloc(self, rhs, sol):
+        """solve() with its workspace in the library's static device array (no allocation)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64).reshape(self.shape3)
+        x = np.array(sol, dtype=np.float64, copy=True).reshape(self.shape3)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0.0)
+        st = _check(lib().ow3d_solve_noalloc(self.h, _p(rhs), _p(x), ctypes.byref(it), ctypes.pointer(res)),
+                    "ow3d_solve_noalloc")
         return x, st, it.value, res.value
 
     def set_surface(self, E, P):
