@@ -62,14 +62,14 @@ def run_steps(T, steps, dt, t0=0.0, status=None):
     return lat
 
 
-def cpu_baseline(cfg_gpu, budget_s=25.0):
-    """The oracle in reference-faithful mode (CSR multigrid with lexicographic Gauss-Seidel, unfused
-    derivative passes, GMRES(23)), one thread, on RK4 steps of the same tank family."""
+def _oracle_tank(workload):
+    """The workload's tank on the oracle in reference-faithful mode: CSR level matrices with the
+    cross-derivative replication, lexicographic Gauss-Seidel with the special-point passes, unfused
+    derivative passes, MGS GMRES(23) at rtol 1e-8."""
     sys.path.insert(0, ROOT)
     import oracle
     from ow3d import problems
-    oracle.build()
-    cfg, bath, E, P, dt, zones = problems.workload("shoal257")
+    cfg, bath, E, P, dt, zones = problems.workload(workload)
     p = dict(nx=cfg["nx"], ny=cfg["ny"], nz=cfg["nz"], alpha=3, beta=3, gamma=3, gridz=1, lx=cfg["lx"], ly=cfg["ly"],
              lz=1.0, g=9.81, nonlinear=1, solver=1, precond=3, gmres_m=23, reltol=1e-8, abstol=0.0, maxit=1, nu1=1,
              nu2=1, mg_maxgrids=20, extrapolation=1, mg_mode=0, cd_quirk=1)
@@ -77,19 +77,55 @@ def cpu_baseline(cfg_gpu, budget_s=25.0):
     for z in zones:
         T.add_relax_zone(*z)
     T.set_state(E, P)
-    steps = 0
+    return T, cfg, dt
+
+
+def _cpu_steps(workload, steps):
+    """Wall seconds and DOF of `steps` RK4 steps of the workload's tank on the oracle (one host thread);
+    run in a child process for the one-tank-per-core figure."""
+    T, cfg, dt = _oracle_tank(workload)
     t0 = time.perf_counter()
-    while True:
-        T.rk4_step(steps * dt, dt)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 2:
-            break
+    for s in range(steps):
+        T.rk4_step(s * dt, dt)
     el = time.perf_counter() - t0
-    dof = cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1)
     T.close()
-    return dict(value=dof * steps / el, unit="DOF-steps/s", cores=1, kind="port",
-                sample=f"{steps} RK4 step(s) of the {cfg['nx']}x{cfg['ny']}x{cfg['nz']} Gaussian-shoal tank (same dx, sponges and "
-                       f"spectrum as the 513 workload, a quarter of its area), reference-faithful oracle, {el:.1f} s on 1 host core")
+    return el, cfg["nx"] * cfg["ny"] * (cfg["nz"] + 1) * steps
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(per_core=True, max_procs=16):
+    """The oracle in reference-faithful mode, one host thread per tank: one RK4 step of the 513x513x9
+    shoal tank itself (value), then, with per_core, one tank per host core (at most max_procs processes,
+    the box's CPU share) each taking one RK4 step of the 257x257x9 member of the same family."""
+    import multiprocessing as mp
+    el, dof_steps = _cpu_steps(WORKLOAD, 1)
+    out = dict(value=dof_steps / el, unit="DOF-steps/s", cores=1, kind="port",
+               sample=f"1 RK4 step of the 513x513x9 Gaussian-shoal tank (the benchmark workload), reference-faithful "
+                      f"oracle (CSR multigrid, lexicographic Gauss-Seidel, GMRES(23)), {el:.1f} s on 1 host core",
+               step_s=el, cpu_model=_cpu_model(), nproc=os.cpu_count())
+    if per_core:
+        procs = max(1, min(max_procs, os.cpu_count() or 1))
+        ctx = mp.get_context("spawn")   # the children never touch the GPU
+        t0 = time.perf_counter()
+        with ctx.Pool(procs) as pool:
+            res = pool.starmap(_cpu_steps, [("shoal257", 1)] * procs)
+        wall = time.perf_counter() - t0
+        slowest = max(r[0] for r in res)
+        out["one_tank_per_core"] = dict(
+            processes=procs, dof_steps_per_s=sum(r[1] for r in res) / slowest, slowest_step_s=slowest,
+            wall_s_incl_setup=wall,
+            sample=f"{procs} processes, each 1 RK4 step of its own 257x257x9 Gaussian-shoal tank (same family and "
+                   f"resolution as the 513 workload); throughput = all tanks' DOF-steps / slowest step")
+    return out
 
 
 def main():
@@ -99,6 +135,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--err-steps", type=int, default=200,
+                    help="RK4 steps of the mixed-vs-FP64 error leg on the 513 workload (0: skip)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-counter runs of the matvec")
     ap.add_argument("--profile-precond", type=int, default=0, metavar="K",
                     help="only apply the FP32 multigrid preconditioner K times to a random vector, then exit")
@@ -122,8 +160,11 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
-    # HBM-counter runs first, as child processes, before this process touches the GPU
-    tr = None if (args.no_traffic or args.profile_matvec or args.profile_precond) else measure_traffic()
+    # HBM-counter runs first, as child processes, before this process touches the GPU; then the CPU baseline
+    # (host only, its one-tank-per-core processes started before the GPU is initialised)
+    side = args.profile_matvec or args.profile_precond or args.batch or args.serve or args.batched_only
+    tr = None if (args.no_traffic or side) else measure_traffic()
+    cpu = None if (args.no_cpu or side) else cpu_baseline()
     import ow3d
     from ow3d import problems
     MG_LEVELS = args.mg_levels
@@ -221,9 +262,32 @@ def main():
     if tr is not None:
         out["roofline"]["traffic"] = tr["bytes"]
         out["roofline"]["traffic_detail"] = tr
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+    if args.err_steps > 0:
+        out["err_leg"] = error_leg(ow3d, problems, args.err_steps)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
+
+
+def error_leg(ow3d, problems, steps):
+    """The 513 workload stepped `steps` times (the configuration's 200) with the FP64 multigrid and with the
+    FP32 multigrid from the same initial state: relative L-inf difference of eta and Phi at the end, steps
+    in which a solve missed its tolerance, and the wall time of each run (one call per step)."""
+    res = {}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        T, cfg, dt = make_tank(ow3d, problems, prec)
+        status = []
+        lat = run_steps(T, steps, dt, status=status)
+        E, P, W, phi = T.get_state()
+        res[label] = dict(E=E, P=P, unconverged_steps=sum(status), wall_s=sum(lat), iters=T.stats())
+        T.close()
+    f, m = res["fp64"], res["mixed"]
+    return {"steps": steps,
+            "rel_err_eta_vs_fp64": float(np.max(np.abs(m["E"] - f["E"])) / np.max(np.abs(f["E"]))),
+            "rel_err_phi_vs_fp64": float(np.max(np.abs(m["P"] - f["P"])) / np.max(np.abs(f["P"]))),
+            "unconverged_steps": {k: v["unconverged_steps"] for k, v in res.items()},
+            "wall_s": {k: round(v["wall_s"], 3) for k, v in res.items()},
+            "iters": {k: v["iters"] for k, v in res.items()}}
 
 
 def bench_batched_513(ow3d, problems, k, steps, warmup):
