@@ -42,11 +42,15 @@ typedef struct ow3d_config {
     int mg_maxgrids;           /* cap on multigrid levels (<= 0: no cap) */
     int extrapolation;         /* extrapolationONOFF: initial guesses from earlier solutions */
     int mg_precision;          /* 0 FP64 multigrid, 1 FP32 multigrid, 2 FP32 multigrid with the coarsest level's dense inverse stored in FP16 */
+    int pressure_term;         /* PressureTermOnOff: applied free-surface pressure of funPressureTerm.f90 (only 0 is supported) */
+    int pressure_damping;      /* PDampingOnOff: pressure damping zones of funPressureTerm.f90 (only 0 is supported) */
 } ow3d_config;
 
 /* Replaces the set-up the reference does in OceanWave3DT0Setup.f90:32-409 for this path
  * (SetupCompDomain, PreProcessDiffStencils, MGPreProcess): grids, stencils and the multigrid
- * hierarchy.  h, hx, hy, hxx, hyy are plane arrays (nxg*nyg); hx.. may be NULL (zero). */
+ * hierarchy.  h, hx, hy, hxx, hyy are plane arrays (nxg*nyg); hx.. may be NULL (zero).
+ * A configuration asking for the applied pressure or pressure damping terms of funPressureTerm.f90
+ * (pressure_term or pressure_damping non-zero; P0, Pd, NuD are zero otherwise) is refused. */
 int  ow3d_create(const ow3d_config *cfg, const double *h, const double *hx, const double *hy,
                  const double *hxx, const double *hyy, ow3d_tank **out);
 void ow3d_destroy(ow3d_tank *t);

@@ -258,6 +258,11 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (c.gmres_m >= MAXK) { err() = "GMRESmaxiterations must be < 32"; return -1; }
     if (c.mg_precision < 0 || c.mg_precision > 2) { err() = "mg_precision must be 0 (FP64), 1 (FP32) or 2 (FP32, FP16 coarsest inverse)"; return -1; }
     if (c.precond != 0 && c.precond != 3) { err() = "precond must be 0 or 3"; return -1; }
+    if (c.pressure_term != 0 || c.pressure_damping != 0) {
+        err() = "applied free-surface pressure and pressure damping (funPressureTerm.f90, PressureTermOnOff / "
+                "PDampingOnOff) are not implemented: pressure_term and pressure_damping must be 0";
+        return -1;
+    }
     auto t = std::make_unique<ow3d_tank>();
     t->cfg = c;
     {

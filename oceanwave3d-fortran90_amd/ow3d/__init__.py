@@ -40,13 +40,13 @@ class Config(ctypes.Structure):
         ("gmres_m", ctypes.c_int), ("reltol", ctypes.c_double), ("abstol", ctypes.c_double),
         ("maxit", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
         ("mg_maxgrids", ctypes.c_int), ("extrapolation", ctypes.c_int),
-        ("mg_precision", ctypes.c_int),
+        ("mg_precision", ctypes.c_int), ("pressure_term", ctypes.c_int), ("pressure_damping", ctypes.c_int),
     ]
 
 
 DEFAULTS = dict(alpha=3, beta=3, gamma=3, gridz=1, lx=1.0, ly=1.0, lz=1.0, g=9.81, nonlinear=1,
                 solver=1, precond=3, gmres_m=23, reltol=1e-8, abstol=0.0, maxit=1, nu1=1, nu2=1,
-                mg_maxgrids=20, extrapolation=1, mg_precision=0)
+                mg_maxgrids=20, extrapolation=1, mg_precision=0, pressure_term=0, pressure_damping=0)
 
 EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_laplace_apply",
            "ow3d_precond_apply", "ow3d_solve", "ow3d_set_surface", "ow3d_vertical_velocity",
