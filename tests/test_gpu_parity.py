@@ -182,6 +182,38 @@ def test_precond_with_a_large_coarsest_level(oracle_mod):
     assert rel(T_g.precond_apply(r), T_o.precond(r)) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("env", [("OW3D_MG_BOTTOM", "4000"), ("OW3D_MG_FUSED_RB", "0")])
+def test_precond_switched_paths_match_the_oracle(oracle_mod, env, prec):
+    """The multigrid paths chosen at handle creation by environment switches: the one-workgroup kernel for
+    the small levels (OW3D_MG_BOTTOM, levels of at most that many points) and the unfused zebra sweeps
+    (separate ghost, red and black launches, OW3D_MG_FUSED_RB=0), on a 65^2 shoal tank whose 4 levels end
+    in a 9^2 x 9 dense solve."""
+    key, val = env
+    os.environ[key] = val
+    try:
+        T_o, T_g, _ = make_pair(oracle_mod, 65, 65, 9, shoal=True, mg_maxgrids=4, mg_precision=prec)
+    finally:
+        del os.environ[key]
+    assert T_g.mg_levels == 4
+    r = np.random.default_rng(12).standard_normal(T_o.shape3)
+    assert rel(T_g.precond_apply(r), T_o.precond(r)) < (1e-10 if prec == 0 else 1e-4)
+
+
+def test_point_matvec_chosen_at_creation_matches_the_oracle(oracle_mod):
+    """OW3D_MV=p selects the one-thread-per-point matvec when the handle is created."""
+    os.environ["OW3D_MV"] = "p"
+    try:
+        T_o, T_g, _ = make_pair(oracle_mod, 29, 23, 9, shoal=True, precond=0)
+    finally:
+        del os.environ["OW3D_MV"]
+    E, P = surface(29, 23, 10.0, seed=13)
+    T_o.surface(E, P)
+    T_g.set_surface(E, P)
+    phi = np.random.default_rng(14).standard_normal(T_o.shape3)
+    assert rel(T_g.laplace_apply(phi), T_o.matvec(phi)) < 1e-12
+
+
 def test_precond_fp32(oracle_mod):
     T_o, T_g, _ = make_pair(oracle_mod, 33, 33, 9, shoal=True, mg_precision=1)
     r = np.random.default_rng(6).standard_normal(T_o.shape3)
