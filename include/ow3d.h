@@ -44,6 +44,9 @@ typedef struct ow3d_config {
     int mg_precision;          /* 0 FP64 multigrid, 1 FP32 multigrid, 2 FP32 multigrid with the coarsest level's dense inverse stored in FP16 */
     int pressure_term;         /* PressureTermOnOff: applied free-surface pressure of funPressureTerm.f90 (only 0 is supported) */
     int pressure_damping;      /* PDampingOnOff: pressure damping zones of funPressureTerm.f90 (only 0 is supported) */
+    int gmres_side;            /* GMRES preconditioning: 0 left (the reference's, iterative_solution.f90:20-46), 1 flexible
+                                  right (FGMRES: stops on the FP64 true residual, whose attainable accuracy does not depend
+                                  on how exactly an FP32 multigrid is linear; takes more iterations) */
 } ow3d_config;
 
 /* Replaces the set-up the reference does in OceanWave3DT0Setup.f90:32-409 for this path
@@ -75,10 +78,11 @@ int ow3d_solve(ow3d_tank *t, const double *rhs, double *sol, int *iters, double 
 /* ow3d_solve that allocates no device memory (no counterpart in the reference; for callers that cannot
  * allocate: tight request loops, harnesses that forbid hipMalloc).  Same arguments, results and return
  * codes as ow3d_solve, bit for bit.  Its workspace -- reduction partial sums, the staged rhs and solution,
- * the residual and correction vectors and the Krylov basis, (GMRESmaxiterations + 5) vectors of the grid
- * -- lives in one static 64 MiB device array per GPU, held until the process ends; a tank whose workspace
- * does not fit (e.g. 257 x 257 x 9 at GMRESmaxiterations 23) is refused with an error.  Every call on a GPU
- * shares that GPU's array: calls that overlap on one GPU (several host threads) race. */
+ * the residual and correction vectors and the Krylov basis (and, with flexible GMRES, the preconditioned
+ * basis), (GMRESmaxiterations + 5) or (2 GMRESmaxiterations + 5) vectors of the grid -- lives in one static
+ * 64 MiB device array per GPU, held until the process ends; a tank whose workspace does not fit (e.g.
+ * 257 x 257 x 9 at GMRESmaxiterations 23) is refused with an error.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (several host threads) race. */
 int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, int *iters, double *res);
 
 /* Replaces DifferentiationsFreeSurfacePlane(Wavefield,...) (timeintegration/DifferentiationsFreeSurfacePlane.f90:1-32)
@@ -113,6 +117,17 @@ int ow3d_stats(const ow3d_tank *t, long long *total, long long *solves, int *itm
  * U, V = horizontal velocities, W = vertical velocity at every grid point (3D layout) from phi (NULL: the
  * handle's current solution) and the handle's current surface. */
 int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, double *V, double *W);
+/* Replaces the velocity-gradient part of StoreKinematicData (IO/StoreKinematicData.f90:327-478): G receives
+ * 9 fields of nxg*nyg*nzg values in the order the reference writes them -- Dsigma(W)/d, (W)_x, (W)_y for
+ * W = dphi/dsigma (that is d w_z, (d w)_x, (d w)_y), then u_z, u_x, u_y, v_z, v_x, v_y -- from phi (NULL: the
+ * handle's current solution) and the handle's current surface.  Uses the Krylov basis as workspace
+ * (GMRESmaxiterations >= 12). */
+int ow3d_kinematics_gradients(ow3d_tank *t, const double *phi, double *G);
+/* The surface slopes eta_x, eta_y (nxg*nyg each) StoreKinematicData writes after eta
+ * (IO/StoreKinematicData.f90:244-255): those of the handle's state, computed as Runge_Kutta_4.f90:394
+ * computes them at the end of each step (and by ow3d_set_state), with the ghost values the surface had
+ * before its own ghost update; ow3d_set_surface replaces them with the slopes of the surface it is given. */
+int ow3d_surface_slopes(ow3d_tank *t, double *Ex, double *Ey);
 
 /* Device-pointer form of ow3d_laplace_apply for callers that keep phi resident (stream: hipStream_t
  * or NULL for the handle's stream).  d_phi must be 16-byte aligned (as hipMalloc returns), d_y 8-byte

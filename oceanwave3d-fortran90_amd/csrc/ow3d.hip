@@ -82,8 +82,9 @@ int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, 
 // vectors, the partial sums of the reductions (RED_BLOCKS x MAXK) and their totals hv (MAXK).
 struct SolveWork {
     double *V, *w, *z, *part, *hv;
+    double *Z;   // flexible GMRES: z_j = M^-1 v_j (m vectors of n)
 };
-SolveWork handle_work(ow3d_tank *t) { return {t->V.p, t->w.p, t->z.p, t->part.p, t->hv.p}; }
+SolveWork handle_work(ow3d_tank *t) { return {t->V.p, t->w.p, t->z.p, t->part.p, t->hv.p, t->Zf.p}; }
 
 // hv[0..k-1] = sum over the partial sums of the last reduction
 void reduce(const SolveWork &W, int k, int nb, const int *skip, hipStream_t s) {
@@ -93,7 +94,9 @@ void reduce(const SolveWork &W, int k, int nb, const int *skip, hipStream_t s) {
 // Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres): classical
 // Gram-Schmidt with the DGKS second pass when the projection removed more than half of the squared norm.
 // At most m operator applications (the initial residual and m - 1 Arnoldi steps), as the reference's
-// ipar(5) = ipar(6) = m setting.
+// ipar(5) = ipar(6) = m setting.  With t->flexible, right preconditioning with the preconditioned vectors
+// kept (FGMRES, Saad 1993): the basis spans A z_j with z_j = M^-1 v_j, x = x0 + Z y, and the stopping test
+// is on the true residual, whose attainable accuracy does not depend on how exactly M^-1 is linear.
 int full_slots(const ow3d_tank *t) {
     if (t->cfg.solver == 0) return std::max(1, t->cfg.gmres_m);
     const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
@@ -108,18 +111,30 @@ int gmres(ow3d_tank *t, const SolveWork &W, const double *b, double *x, hipStrea
     KrylovState *S = t->ks.p;
     const int *done = t->done(), *skip2 = t->skip2();
     const int nbl = grid_blocks(n), nb = red_blocks(n);
+    const bool flex = t->flexible;
     hipLaunchKernelGGL(k_solve_begin, dim3(1), dim3(64), 0, s, S);
-    if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
-    hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
-    if (precond(t, w, V, s)) return -1;
+    if (flex) {   // v0 = b - A x0
+        if (launch_matvec(t, x, V, s, done)) { err() = "matvec dispatch failed"; return -1; }
+        hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, V, done);
+    } else {      // v0 = M^-1 (b - A x0)
+        if (launch_matvec(t, x, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+        hipLaunchKernelGGL(k_bminus, dim3(nbl), dim3(256), 0, s, n, b, w, done);
+        if (precond(t, w, V, s)) return -1;
+    }
     if (launch_mdot(1, false, nb, n, V, V, W.part, done, s)) return -1;
     reduce(W, 1, nb, done, s);
     hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, t->cfg.reltol, t->cfg.abstol);
     hipLaunchKernelGGL(k_scale_state, dim3(nbl), dim3(256), 0, s, n, (const KrylovState *)S, V);
     for (int k = 1; k <= slots; k++) {
         double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
-        if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
-        if (precond(t, w, vn, s)) return -1;
+        if (flex) {   // z = M^-1 v_k-1, kept; the new direction is A z
+            double *zk = W.Z + (size_t)(k - 1) * n;
+            if (precond(t, vk, zk, s)) return -1;
+            if (launch_matvec(t, zk, vn, s, done)) { err() = "matvec dispatch failed"; return -1; }
+        } else {
+            if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+            if (precond(t, w, vn, s)) return -1;
+        }
         if (launch_mdot(k, true, nb, n, V, vn, W.part, done, s)) return -1;
         reduce(W, k + 1, nb, done, s);
         hipLaunchKernelGGL(k_gmres_pass1, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, k, m);
@@ -130,7 +145,7 @@ int gmres(ow3d_tank *t, const SolveWork &W, const double *b, double *x, hipStrea
         if (launch_gs_update(k, nb, n, V, W.hv, &S->scale, vn, done, s)) return -1;
     }
     hipLaunchKernelGGL(k_gmres_finish, dim3(1), dim3(64), 0, s, S, slots, full_slots(t), stage);
-    hipLaunchKernelGGL(k_basis_combine, dim3(nbl), dim3(256), 0, s, n, (const double *)V, (const KrylovState *)S, x);
+    hipLaunchKernelGGL(k_basis_combine, dim3(nbl), dim3(256), 0, s, n, (const double *)(flex ? W.Z : V), (const KrylovState *)S, x);
     return 0;
 }
 
@@ -258,6 +273,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     if (c.gmres_m >= MAXK) { err() = "GMRESmaxiterations must be < 32"; return -1; }
     if (c.mg_precision < 0 || c.mg_precision > 2) { err() = "mg_precision must be 0 (FP64), 1 (FP32) or 2 (FP32, FP16 coarsest inverse)"; return -1; }
     if (c.precond != 0 && c.precond != 3) { err() = "precond must be 0 or 3"; return -1; }
+    if (c.gmres_side < 0 || c.gmres_side > 1) { err() = "gmres_side must be 0 (left) or 1 (flexible right)"; return -1; }
     if (c.pressure_term != 0 || c.pressure_damping != 0) {
         err() = "applied free-surface pressure and pressure damping (funPressureTerm.f90, PressureTermOnOff / "
                 "PDampingOnOff) are not implemented: pressure_term and pressure_damping must be 0";
@@ -341,6 +357,11 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     HIPCHK(t->hv.alloc(MAXK));
     HIPCHK(t->V.alloc((size_t)(m + 1) * n));
     HIPCHK(hipMemset(t->V.p, 0, (size_t)(m + 1) * n * 8));   // unused basis vectors stay finite
+    t->flexible = c.solver == 1 && c.gmres_side == 1;
+    if (t->flexible) {
+        HIPCHK(t->Zf.alloc((size_t)m * n));
+        HIPCHK(hipMemset(t->Zf.p, 0, (size_t)m * n * 8));
+    }
     {
         KrylovState k0{};
         k0.it_min = -1;
@@ -464,7 +485,8 @@ extern "C" int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, 
     const size_t n = t->g.n, np = pad32(n);
     const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
     const size_t head = pad32((size_t)RED_BLOCKS * MAXK) + pad32(MAXK);
-    const size_t need = head + (size_t)(4 + m + 1) * np;
+    const size_t nbasis = (size_t)(m + 1) + (t->flexible ? (size_t)m : 0);   // V, then Z
+    const size_t need = head + (4 + nbasis) * np;
     if (need > SCRATCH_DOUBLES) {
         err() = "tank too large for the no-allocation solve: its workspace needs " + std::to_string(need * 8 >> 20) +
                 " MiB, the static array holds " + std::to_string(SCRATCH_BYTES >> 20) + " MiB";
@@ -473,10 +495,11 @@ extern "C" int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, 
     double *base = nullptr;
     if (scratch_ptr(&base)) return -1;
     double *b = base + head, *x = b + np;
-    SolveWork W{x + 3 * np, x + np, x + 2 * np, base, base + pad32((size_t)RED_BLOCKS * MAXK)};
+    SolveWork W{x + 3 * np, x + np, x + 2 * np, base, base + pad32((size_t)RED_BLOCKS * MAXK),
+                t->flexible ? x + 3 * np + (size_t)(m + 1) * np : nullptr};
     hipStream_t s = t->s;
-    // the array keeps the last call's contents: the basis starts from zeros, as the handle's does
-    hipLaunchKernelGGL(k_zero, dim3(grid_blocks((size_t)(m + 1) * np)), dim3(256), 0, s, (size_t)(m + 1) * np, W.V);
+    // the array keeps the last call's contents: the bases start from zeros, as the handle's do
+    hipLaunchKernelGGL(k_zero, dim3(grid_blocks(nbasis * np)), dim3(256), 0, s, nbasis * np, W.V);
     HIPCHK(hipMemcpyAsync(b, rhs, n * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(x, sol, n * 8, hipMemcpyHostToDevice, s));
     if (solve_dev(t, b, x, s, -1, 0, &W)) return -1;
@@ -872,12 +895,50 @@ extern "C" int ow3d_kinematics(ow3d_tank *t, const double *phi, double *U, doubl
     a.sx = t->sx.p; a.sy = t->sy.p; a.sz = t->sz.p; a.z = t->zr.p; a.h = t->h.p; a.bath = t->bath.p;
     a.E = t->E.p; a.Ex = t->Ex.p; a.Ey = t->Ey.p;
     // outputs: U in io_b, V in w, W in z (scratch vectors of the handle, free outside a solve)
-    hipLaunchKernelGGL(k_kinematics, dim3(grid_blocks(n)), dim3(256), 0, s, a, src, t->io_b.p, t->w.p, t->z.p);
+    hipLaunchKernelGGL(k_kinematics, dim3(grid_blocks(n)), dim3(256), 0, s, a, src, t->io_b.p, t->w.p, t->z.p, (double *)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(U, t->io_b.p, n * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(V, t->w.p, n * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(W, t->z.p, n * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int ow3d_kinematics_gradients(ow3d_tank *t, const double *phi, double *G) {
+    if (!t || !G) { err() = "null argument"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    const size_t n = t->g.n;
+    if ((size_t)t->cfg.gmres_m + 1 < 13) { err() = "the kinematics gradients need GMRESmaxiterations >= 12 (they use the Krylov basis as workspace)"; return -1; }
+    std::lock_guard<std::mutex> lk(capture_lock());
+    hipStream_t s = t->s;
+    const double *src = t->phi.p;
+    if (phi) {
+        HIPCHK(hipMemcpyAsync(t->io_a.p, phi, n * 8, hipMemcpyHostToDevice, s));
+        src = t->io_a.p;
+    }
+    KinArgs a{};
+    a.nxg = t->g.nxg; a.nyg = t->g.nyg; a.nzg = t->g.nzg; a.a = t->g.a; a.b = t->g.b; a.c = t->g.c;
+    a.nonlinear = t->cfg.nonlinear;
+    a.sx = t->sx.p; a.sy = t->sy.p; a.sz = t->sz.p; a.z = t->zr.p; a.h = t->h.p; a.bath = t->bath.p;
+    a.E = t->E.p; a.Ex = t->Ex.p; a.Ey = t->Ey.p;
+    // workspace: the Krylov basis (free outside a solve) holds U, V, w, W and the 9 gradient fields
+    double *U = t->V.p, *V = U + n, *Wv = V + n, *Ws = Wv + n, *Gd = Ws + n;
+    hipLaunchKernelGGL(k_kinematics, dim3(grid_blocks(n)), dim3(256), 0, s, a, src, U, V, Wv, Ws);
+    hipLaunchKernelGGL(k_kinematics_grad, dim3(grid_blocks(n)), dim3(256), 0, s, a, (const double *)Ws, (const double *)U,
+                       (const double *)V, Gd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(G, Gd, 9 * n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int ow3d_surface_slopes(ow3d_tank *t, double *Ex, double *Ey) {
+    if (!t || !Ex || !Ey) { err() = "null argument"; return -1; }
+    if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
+    const size_t nxy = t->g.nxy;
+    HIPCHK(hipMemcpyAsync(Ex, t->Ex.p, nxy * 8, hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipMemcpyAsync(Ey, t->Ey.p, nxy * 8, hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
     return 0;
 }
 
@@ -920,6 +981,8 @@ extern "C" long long ow3d_estimate_bytes(const ow3d_config *cfg) {
     const long long n = (long long)g.n, nxy = g.nxy;
     const int m = cfg->gmres_m > 1 ? cfg->gmres_m : 15;
     long long b = 8 * n * (8 + m + 1 + 1) + 8 * nxy * (25 + 3);   // + step snapshot (phi, E, P, W)
+    const bool flex = cfg->solver == 1 && cfg->gmres_side == 1;
+    if (flex) b += 8 * n * m;                                        // preconditioned basis of flexible GMRES
     b += 8LL * ((long long)g.nxg * 2 * g.ra + (cfg->ny > 1 ? (long long)g.nyg * 2 * g.rb : 6) + 2LL * g.nzg * 2 * g.rc);
     b += 8LL * RED_BLOCKS * MAXK + 8LL * MAXK + (long long)sizeof(KrylovState);
     if (cfg->precond == 3) {

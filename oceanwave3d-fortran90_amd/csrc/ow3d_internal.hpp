@@ -111,6 +111,8 @@ struct ow3d_tank {
     DevBuf<double> E, P, W, Ex, Exx, Ey, Eyy, Px, Py, Et, Pt, Wt, KE, KP;
     // Laplace solve
     DevBuf<double> phi, rhs, last0, last1, w, z, V, part, red, hv;
+    DevBuf<double> Zf;                  // flexible GMRES: the preconditioned basis vectors
+    bool flexible = false;              // right-preconditioned (flexible) GMRES instead of left
     DevBuf<double> io_a, io_b;   // staging for host-pointer entry points
     // relaxation
     std::vector<Zone> zones;

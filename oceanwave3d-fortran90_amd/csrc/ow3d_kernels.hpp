@@ -472,12 +472,13 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
 }
 
 // The column matvec with the work of each column split between two waves: a block of 16 x 16 columns
-// has 512 threads; threads 0..255 (waves 0-3) take the x-side of column t (d/dx, d2/dx2, the sigma
-// derivatives of phi and the mixed x-sigma term, the bottom row), threads 256..511 (waves 4-7) the
-// y-side of column t - 256 (d/dy, d2/dy2, the mixed y-sigma term).  The y-side leaves its part of every
-// row in LDS; the x-side adds it.  Each thread holds half the live values of the one-thread-per-column
-// kernel and a block has twice the waves.  Same terms as k_matvec3c; the x and y parts are summed in
-// another order (results agree to rounding).
+// has 512 threads; threads 0..255 (waves 0-3) take the x-side of column t (d/dx, d2/dx2, the first sigma
+// derivative of phi and the mixed x-sigma term, the bottom row), threads 256..511 (waves 4-7) the
+// y-side of column t - 256 (d/dy, d2/dy2, the second sigma derivative and the mixed y-sigma term): four
+// 2A+1-point stencils per level on each side.  The y-side leaves its part of every row in LDS; the x-side
+// adds it.  Each thread holds half the live values of the one-thread-per-column kernel and a block has
+// twice the waves.  Same terms as k_matvec3c; the x and y parts are summed in another order (results agree
+// to rounding).
 template <int NZ, int A>
 __global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
     if (solve_done(g.skip)) return;
@@ -576,16 +577,26 @@ __global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__rest
                 }
             }
             const double *cfp = g.cf + i + (size_t)nxg * j;
-            const double A4 = cfp[2 * (size_t)nxy], B4 = cfp[3 * (size_t)nxy], S5 = cfp[6 * (size_t)nxy];
+            const double A2 = cfp[0], B2 = cfp[nxy], A4 = cfp[2 * (size_t)nxy], B4 = cfp[3 * (size_t)nxy];
+            const double S5 = cfp[6 * (size_t)nxy];
+            // the second sigma derivative is this side's share of the column's own terms (the x side takes
+            // the first), so that both halves of a column do the same work
+            double pc[NZ];
+#pragma unroll
+            for (int k = 0; k < NZ; k++) pc[k] = P(k, i, j);
 #pragma unroll
             for (int k = 1; k < NZ - 1; k++) {
                 const int wz = cclamp(k - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (k * 2) * RA;
-                double dyz = 0.0;
+                const double *Z1 = g.sz + (k * 2) * RA, *Z2 = Z1 + RA;
+                double dyz = 0.0, dzz = 0.0;
 #pragma unroll
-                for (int m = 0; m < RA; m++) dyz += Z1[m] * ((wz + m == 0) ? dyc0 : dy[wz + m]);
-                const double s4 = A4 + g.zs[k] * B4;
-                o[k] = o[k] + 2.0 * (s4 * dyz);
+                for (int m = 0; m < RA; m++) {
+                    dyz += Z1[m] * ((wz + m == 0) ? dyc0 : dy[wz + m]);
+                    dzz += Z2[m] * pc[wz + m];
+                }
+                const double sg = g.zs[k];
+                const double s2 = A2 + sg * B2, s4 = A4 + sg * B4;
+                o[k] = o[k] + 2.0 * (s4 * dyz) + (s2 * s2 + s4 * s4 + S5 * S5) * dzz;
             }
             const double d = 1.0 / S5;
             o[0] = (A4 * d) * dy[1];   // h_y dy at the bottom row (h_y = A4 d, d = 1 / S5)
@@ -651,18 +662,17 @@ __global__ __launch_bounds__(512) void k_matvec3s(MvArgs g, const double *__rest
 #pragma unroll
             for (int k = 1; k < NZ - 1; k++) {
                 const int wz = cclamp(k - A, 0, NZ - RA);
-                const double *Z1 = g.sz + (k * 2) * RA, *Z2 = Z1 + RA;
-                double dz = 0.0, dzz = 0.0, dxz = 0.0;
+                const double *Z1 = g.sz + (k * 2) * RA;
+                double dz = 0.0, dxz = 0.0;
 #pragma unroll
                 for (int m = 0; m < RA; m++) {
-                    const double z1 = Z1[m], cv = pc[wz + m];
-                    dz += z1 * cv;
-                    dzz += Z2[m] * cv;
+                    const double z1 = Z1[m];
+                    dz += z1 * pc[wz + m];
                     dxz += z1 * ((wz + m == 0) ? dxc0 : dx[wz + m]);
                 }
                 const double sg = g.zs[k];
-                const double s2 = A2 + sg * B2, s4 = A4 + sg * B4, s3 = A3 + sg * B3;
-                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz) + (s2 * s2 + s4 * s4 + S5 * S5) * dzz;
+                const double s2 = A2 + sg * B2, s3 = A3 + sg * B3;
+                o[k] = o[k] + s3 * dz + 2.0 * (s2 * dxz);
             }
             o[NZ - 1] = pc[NZ - 1];
         }
@@ -934,7 +944,8 @@ struct KinArgs {
     const double *E, *Ex, *Ey;     // surface and its derivatives
 };
 static __global__ __launch_bounds__(256) void k_kinematics(KinArgs a, const double *__restrict__ phi, double *__restrict__ U,
-                                                           double *__restrict__ V, double *__restrict__ Wv) {
+                                                           double *__restrict__ V, double *__restrict__ Wv,
+                                                           double *__restrict__ Ws = nullptr) {
     const size_t nxy = (size_t)a.nxg * a.nyg, n = nxy * a.nzg;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(q % a.nzg);
@@ -961,6 +972,50 @@ static __global__ __launch_bounds__(256) void k_kinematics(KinArgs a, const doub
             if (a.nyg > 1) v = v + ((1 - z) / d * a.bath[nxy + p] - z / d * a.Ey[p]) * w;
         }
         U[q] = u; V[q] = v; Wv[q] = w / d;
+        if (Ws) Ws[q] = w;
+    }
+}
+
+// The velocity gradients StoreKinematicData.f90:327-478 writes after u, v, w, from F = (W, U, V) with
+// W = dphi/dsigma (not divided by d) and U, V the velocities above: for each field, G = Dsigma(F) / d,
+// Dx(F) + cx Dsigma(F), Dy(F) + cy Dsigma(F) with the chain-rule factors cx = (1 - z)/d h_x - z/d eta_x,
+// cy likewise (nonlinear mode only), x (y) derivatives zero when the grid has one column (row).  G holds
+// the 9 fields of n points in that order (for W: d w_z, (d w)_x, (d w)_y).  One thread per point.
+static __global__ __launch_bounds__(256) void k_kinematics_grad(KinArgs a, const double *__restrict__ W,
+                                                                const double *__restrict__ U, const double *__restrict__ V,
+                                                                double *__restrict__ G) {
+    const size_t nxy = (size_t)a.nxg * a.nyg, n = nxy * a.nzg;
+    const double *F[3] = {W, U, V};
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q % a.nzg);
+        const size_t p = q / a.nzg;
+        const int i = (int)(p % a.nxg), j = (int)(p / a.nxg);
+        const int rz = 2 * a.c + 1, wz = cclamp(k - a.c, 0, a.nzg - rz);
+        const double *Sz = a.sz + (size_t)(k * 2) * rz;
+        const double d = a.h[p] + a.E[p], z = a.z[k];
+        const double cx = ((1 - z) / d * a.bath[p] - z / d * a.Ex[p]);
+        const double cy = ((1 - z) / d * a.bath[nxy + p] - z / d * a.Ey[p]);
+        for (int f = 0; f < 3; f++) {
+            const double *Fv = F[f];
+            double fz = 0.0;
+            for (int m = 0; m < rz; m++) fz += Sz[m] * Fv[p * a.nzg + wz + m];
+            double fx = 0.0, fy = 0.0;
+            if (a.nxg > 1) {
+                const int rx = 2 * a.a + 1, wx = cclamp(i - a.a, 0, a.nxg - rx);
+                const double *Sx = a.sx + (size_t)(i * 2) * rx;
+                for (int m = 0; m < rx; m++) fx += Sx[m] * Fv[((size_t)j * a.nxg + wx + m) * a.nzg + k];
+                if (a.nonlinear) fx = fx + cx * fz;
+            }
+            if (a.nyg > 1) {
+                const int ry = 2 * a.b + 1, wy = cclamp(j - a.b, 0, a.nyg - ry);
+                const double *Sy = a.sy + (size_t)(j * 2) * ry;
+                for (int m = 0; m < ry; m++) fy += Sy[m] * Fv[((size_t)(wy + m) * a.nxg + i) * a.nzg + k];
+                if (a.nonlinear) fy = fy + cy * fz;
+            }
+            G[(size_t)(3 * f) * n + q] = fz / d;
+            G[(size_t)(3 * f + 1) * n + q] = fx;
+            G[(size_t)(3 * f + 2) * n + q] = fy;
+        }
     }
 }
 
@@ -1354,6 +1409,10 @@ struct LvArgs {
     const int *skip;        // solve_done flag (nullptr: always run)
     int spec;               // 1: the kernels issue their loads before testing skip (small levels, where a
                             // launch is a chain of memory latencies); 0: skip is tested first
+    // uniform horizontal grid (uni = 1): every interior column's centred 3-point x (y) rows are the same,
+    // first derivative (-cx1, 0, cx1), second derivative (cx2, cxc, cx2)
+    int uni;
+    T cx1, cx2, cxc, cy1, cy2, cyc;
 };
 
 // Level coefficients as the column kernels read them: 3-point stencil rows sx(i), sy(j), sz(k) (first then
@@ -1829,6 +1888,34 @@ __device__ __forceinline__ void zebra_row_interior(const CA &C, bool d3, int i, 
     rh = bk - off;
 }
 
+// Row k (1 <= k <= nz - 2) of an interior column (2 <= i <= nxg - 3, 2 <= j <= nyg - 3) on a uniform
+// horizontal grid: zebra_row_interior with the level's constant centred stencils, so each mixed-derivative
+// coupling is 2 s2 cx1 sum_m Z1[m] (x[i+1] - x[i-1]) on the three sigma levels around k (the centre
+// weight of the first derivative is zero).  c points at the column's level 0 in LDS.
+template <typename T, int SXS, int SYS, class CA>
+__device__ __forceinline__ void zebra_row_uniform(const LvArgs<T> &L, const CA &C, bool d3, int i, int j, int k, const T *c,
+                                                  T bk, T &lo, T &di, T &up, T &rh) {
+    const T *Z1 = C.sz(k), *Z2 = Z1 + 3;
+    const T om = T(1) - C.zs(k);
+    const T s2 = om * C.col(3, i, j), s3 = om * C.col(5, i, j), s5 = C.col(0, i, j);
+    const T *xm = c - SXS, *xp = c + SXS;
+    const T gx = (Z1[0] * (xp[k - 1] - xm[k - 1]) + Z1[1] * (xp[k] - xm[k])) + Z1[2] * (xp[k + 1] - xm[k + 1]);
+    T off = L.cx2 * (xm[k] + xp[k]) + (T(2) * L.cx1) * s2 * gx;
+    T czz = s2 * s2 + s5 * s5, dc = L.cxc;
+    if (d3) {
+        const T s4 = om * C.col(4, i, j);
+        const T *ym = c - SYS, *yp = c + SYS;
+        const T gy = (Z1[0] * (yp[k - 1] - ym[k - 1]) + Z1[1] * (yp[k] - ym[k])) + Z1[2] * (yp[k + 1] - ym[k + 1]);
+        off += L.cy2 * (ym[k] + yp[k]) + (T(2) * L.cy1) * s4 * gy;
+        czz += s4 * s4;
+        dc += L.cyc;
+    }
+    lo = s3 * Z1[0] + czz * Z2[0];
+    di = (s3 * Z1[1] + czz * Z2[1]) + dc;
+    up = s3 * Z1[2] + czz * Z2[2];
+    rh = bk - off;
+}
+
 // Residual o = b - L x of column (i, j) of a level (all row kinds: interior, bottom ghost, surface,
 // wall ghost columns, corners), x read through X(k, i, j).
 template <typename T, int NZ, class XA, class CA>
@@ -2142,9 +2229,15 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
             if (sl >= ns || !slot(sl, i, j)) continue;
             T lo, di, up, rh, e2;
             const T bk = sB[((j - bloy) * (TX + 2) + (i - blox)) * NZ + k];
-            if (i >= 2 && i <= nxg - 3 && (!d3 || (j >= 2 && j <= nyg - 3)))
-                zebra_row_interior<T, NZP, WX * NZP>(C, d3, i, j, k, NZ, sX + ((j - loy) * WX + (i - lox)) * NZP, bk, lo, di, up, rh, e2);
-            else
+            if (i >= 2 && i <= nxg - 3 && (!d3 || (j >= 2 && j <= nyg - 3))) {
+                const T *cc = sX + ((j - loy) * WX + (i - lox)) * NZP;
+                if (L.uni && k >= 1 && k <= NZ - 2) {
+                    zebra_row_uniform<T, NZP, WX * NZP>(L, C, d3, i, j, k, cc, bk, lo, di, up, rh);
+                    e2 = T(0);
+                } else {
+                    zebra_row_interior<T, NZP, WX * NZP>(C, d3, i, j, k, NZ, cc, bk, lo, di, up, rh, e2);
+                }
+            } else
                 zebra_row<T>(L, C, i, j, k, NZ, X, bk, lo, di, up, rh, e2);
             sR[(0 * NZ + k) * NS + sl] = lo; sR[(1 * NZ + k) * NS + sl] = di;
             sR[(2 * NZ + k) * NS + sl] = up; sR[(3 * NZ + k) * NS + sl] = rh;

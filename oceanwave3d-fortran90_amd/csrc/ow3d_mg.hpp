@@ -180,6 +180,23 @@ inline void low_apply_host(const Grid &g, const std::vector<double> &x, std::vec
     }
 }
 
+// True when the 3-point rows S[N][2][3] of nodes 2 .. N-3 are the centred rows of a uniform grid: all
+// equal (to 1e-12) to the middle node's, with a zero centre weight in the first derivative.  c receives
+// the first-derivative weight of the right neighbour, the second-derivative neighbour and centre weights.
+inline bool uniform_rows(const std::vector<double> &S, int N, double *c) {
+    if (N < 5) return false;
+    const double *R = &S[(size_t)(N / 2) * 6];
+    c[0] = R[2]; c[1] = R[3]; c[2] = R[4];
+    const double t1 = 1e-12 * std::fabs(c[0]), t2 = 1e-12 * std::fabs(c[2]);
+    for (int i = 2; i <= N - 3; i++) {
+        const double *Q = &S[(size_t)i * 6];
+        if (std::fabs(Q[0] + c[0]) > t1 || std::fabs(Q[1]) > t1 || std::fabs(Q[2] - c[0]) > t1 ||
+            std::fabs(Q[3] - c[1]) > t2 || std::fabs(Q[4] - c[2]) > t2 || std::fabs(Q[5] - c[1]) > t2)
+            return false;
+    }
+    return true;
+}
+
 template <typename T>
 int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
     hipStream_t s = t->s;
@@ -252,6 +269,13 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         // bytes: its kernels issue their loads before testing skip (one wait instead of two), at the cost
         // of reading the level once more in launches that follow convergence
         L->args.spec = g.n <= (1u << 20) ? 1 : 0;
+        {   // uniform horizontal grid: the interior columns' centred rows are all equal to the middle one's
+            double c[6] = {0, 0, 0, 0, 0, 0};
+            const bool ux = uniform_rows(g.sx, g.nxg, c), uy = g.ny == 1 || uniform_rows(g.sy, g.nyg, c + 3);
+            L->args.uni = ux && uy;
+            L->args.cx1 = (T)c[0]; L->args.cx2 = (T)c[1]; L->args.cxc = (T)c[2];
+            L->args.cy1 = (T)c[3]; L->args.cy2 = (T)c[4]; L->args.cyc = (T)c[5];
+        }
         mg.push_back(std::move(L));
     }
     (void)s;

@@ -27,6 +27,7 @@ CONTAINS
     c%reltol = reltol; c%abstol = abstol; c%maxit = maxit; c%nu1 = nu(1); c%nu2 = nu(2)
     c%mg_maxgrids = MGmaxgrids; c%extrapolation = extrapolationONOFF; c%mg_precision = 0
     c%pressure_term = PressureTermOnOff; c%pressure_damping = PDampingOnOff
+    c%gmres_side = 0   ! left preconditioning, as iterative_solution.f90:20-46
     IF (Precond == 3 .AND. cyclet /= 'V') STOP 'ow3d: only V cycles are implemented (cyclet must be V)'
     rc = ow3d_create(c, FineGrid%h, FineGrid%hx, FineGrid%hy, FineGrid%hxx, FineGrid%hyy, ow3d_handle)
     IF (rc /= 0) STOP 'ow3d_create failed'

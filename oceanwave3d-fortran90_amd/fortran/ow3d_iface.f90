@@ -12,7 +12,7 @@ MODULE ow3d_iface
      INTEGER(C_INT)  :: nonlinear, solver, precond, gmres_m
      REAL(C_DOUBLE)  :: reltol, abstol
      INTEGER(C_INT)  :: maxit, nu1, nu2, mg_maxgrids, extrapolation, mg_precision
-     INTEGER(C_INT)  :: pressure_term, pressure_damping
+     INTEGER(C_INT)  :: pressure_term, pressure_damping, gmres_side
   END TYPE ow3d_config
 
   INTERFACE

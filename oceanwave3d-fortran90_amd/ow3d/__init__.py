@@ -41,12 +41,14 @@ class Config(ctypes.Structure):
         ("maxit", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
         ("mg_maxgrids", ctypes.c_int), ("extrapolation", ctypes.c_int),
         ("mg_precision", ctypes.c_int), ("pressure_term", ctypes.c_int), ("pressure_damping", ctypes.c_int),
+        ("gmres_side", ctypes.c_int),
     ]
 
 
 DEFAULTS = dict(alpha=3, beta=3, gamma=3, gridz=1, lx=1.0, ly=1.0, lz=1.0, g=9.81, nonlinear=1,
                 solver=1, precond=3, gmres_m=23, reltol=1e-8, abstol=0.0, maxit=1, nu1=1, nu2=1,
-                mg_maxgrids=20, extrapolation=1, mg_precision=0, pressure_term=0, pressure_damping=0)
+                mg_maxgrids=20, extrapolation=1, mg_precision=0, pressure_term=0, pressure_damping=0,
+                gmres_side=0)
 
 EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_laplace_apply",
            "ow3d_precond_apply", "ow3d_solve", "ow3d_set_surface", "ow3d_vertical_velocity",
@@ -54,7 +56,7 @@ EXPORTS = ["ow3d_create", "ow3d_destroy", "ow3d_dims", "ow3d_last_error", "ow3d_
            "ow3d_rk4_steps", "ow3d_stats", "ow3d_laplace_apply_device", "ow3d_stream",
            "ow3d_time_matvec", "ow3d_batch_rk4_steps", "ow3d_rk4_steps_async", "ow3d_query", "ow3d_wait",
            "ow3d_device_bytes", "ow3d_estimate_bytes", "ow3d_step_stats", "ow3d_tune", "ow3d_mg_plan", "ow3d_kinematics", "ow3d_steps_status",
-           "ow3d_solve_noalloc"]
+           "ow3d_solve_noalloc", "ow3d_kinematics_gradients", "ow3d_surface_slopes"]
 
 
 class OW3DError(RuntimeError):
@@ -102,6 +104,8 @@ def lib():
         L.ow3d_device_bytes.argtypes = [P]
         L.ow3d_tune.argtypes = [P, I, I]
         L.ow3d_kinematics.argtypes = [P, D, D, D, D]
+        L.ow3d_kinematics_gradients.argtypes = [P, D, D]
+        L.ow3d_surface_slopes.argtypes = [P, D, D]
         L.ow3d_mg_plan.argtypes = [ctypes.POINTER(Config), IP, IP]
         L.ow3d_step_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), D]
         L.ow3d_estimate_bytes.restype = ctypes.c_longlong
@@ -279,6 +283,34 @@ class Tank:
         U, V, W = (np.empty(self.shape3) for _ in range(3))
         _check(lib().ow3d_kinematics(self.h, _p(p), _p(U), _p(V), _p(W)), "ow3d_kinematics")
         return U, V, W
+
+    def kinematics_gradients(self, phi=None):
+        """The 9 fields StoreKinematicData.f90:327-478 writes after u, v, w, shape (9,) + shape3: d w_z,
+        (d w)_x, (d w)_y, u_z, u_x, u_y, v_z, v_x, v_y."""
+        p = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape3)
+        G = np.empty((9,) + self.shape3)
+        _check(lib().ow3d_kinematics_gradients(self.h, _p(p), _p(G)), "ow3d_kinematics_gradients")
+        return G
+
+    def store_kinematics(self, directory, io, it, win, dt, grid=None):
+        """Writes Kinematics_<io>_<it>.bin (formattype 21, StoreKinematicData.f90:90-478) for the tank's
+        current state: with it == 0 the grid file (grid = dict(x, y, h, hx, hy, z), the tank's set-up
+        fields), else eta and its slopes, phi, u, v, w and the velocity gradients of the window."""
+        from ow3d import io as owio
+        if it == 0:
+            return owio.write_kinematics(directory, io, 0, win, dt, grid=grid)
+        E, P, W, phi = self.get_state()
+        Ex, Ey = self.surface_slopes()
+        U, V, Wv = self.kinematics()
+        G = self.kinematics_gradients()
+        return owio.write_kinematics(directory, io, it, win, dt,
+                                     fields=dict(E=E, Ex=Ex, Ey=Ey, phi=phi, U=U, V=V, W=Wv, G=G))
+
+    def surface_slopes(self):
+        """(eta_x, eta_y) of the tank's state, as the last time step or set_state computed them."""
+        Ex, Ey = np.empty(self.shape2), np.empty(self.shape2)
+        _check(lib().ow3d_surface_slopes(self.h, _p(Ex), _p(Ey)), "ow3d_surface_slopes")
+        return Ex, Ey
 
     def tune(self, matvec_variant=0, sweep_threads=0):
         """Launch choices (see ow3d_tune in include/ow3d.h); results do not depend on them."""

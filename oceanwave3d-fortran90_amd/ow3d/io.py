@@ -14,6 +14,8 @@ reference run from this one:
 Plane arrays here are numpy arrays of shape (nyg, nxg) with the ghost layer, as everywhere in the
 package; files hold interior nodes only where the reference's do.
 """
+import os
+
 import numpy as np
 
 
@@ -149,3 +151,43 @@ def read_ep(path):
     m = nxg * nyg
     sh = (nyg, nxg)
     return xy[:m].reshape(sh), xy[m:].reshape(sh), ep[:m].reshape(sh), ep[m:].reshape(sh)
+
+
+def kinematics_path(directory, io, it):
+    """Kinematics_<io:2>_<it:5>.bin, the name StoreKinematicData.f90:91-96 gives formattype-21 files."""
+    return os.path.join(directory, "Kinematics_%02d_%05d.bin" % (io, it))
+
+
+def write_kinematics(directory, io, it, win, dt, grid=None, fields=None):
+    """Writes one formattype-21 kinematics file as StoreKinematicData.f90:90-478 does (unformatted sequential
+    records) and returns its path.  win = (xbeg, xend, xstride, ybeg, yend, ystride, tbeg, tend, tstride) with
+    the reference's 1-based interior node numbers.
+    it == 0: the grid file -- the window and time stride, dt and the column depth; x, y, h, h_x, h_y of every
+    window column; the sigma levels.  grid = dict(x, y, h, hx, hy (ghost-extended planes), z (sigma levels)).
+    it > 0: eta, eta_x, eta_y (zero when the grid has one column / row) of the window, then phi, u, v, w and the
+    9 gradient fields (ow3d_kinematics_gradients order) of its columns.  fields = dict(E, Ex, Ey (planes),
+    phi, U, V, W (3D, W the vertical velocity), G (9 x 3D))."""
+    xb, xe, xs, yb, ye, ys, tb, te, ts = (int(v) for v in win)
+    ref = grid if it == 0 else fields
+    nyg, nxg = np.asarray(ref["h" if it == 0 else "E"]).shape
+    gx, gy = 1, (1 if nyg > 1 else 0)
+    si = slice(xb + gx - 1, xe + gx, xs)
+    sj = slice(yb + gy - 1, ye + gy, ys)
+    path = kinematics_path(directory, io, it)
+    with open(path, "wb") as f:
+        if it == 0:
+            nz = len(grid["z"])
+            _record(f, np.array([xb, xe, xs, yb, ye, ys, tb, te, ts], dtype=np.int32), np.array([dt], dtype=np.float64),
+                    np.array([nz], dtype=np.int32))
+            cols = np.stack([np.asarray(grid[k], dtype=np.float64)[sj, si] for k in ("x", "y", "h", "hx", "hy")], axis=-1)
+            _record(f, cols)
+            _record(f, np.asarray(grid["z"], dtype=np.float64))
+            return path
+        E = np.asarray(fields["E"], dtype=np.float64)
+        _record(f, E[sj, si])
+        _record(f, np.asarray(fields["Ex"], dtype=np.float64)[sj, si] if nxg > 1 else np.zeros_like(E[sj, si]))
+        _record(f, np.asarray(fields["Ey"], dtype=np.float64)[sj, si] if nyg > 1 else np.zeros_like(E[sj, si]))
+        vols = [fields["phi"], fields["U"], fields["V"], fields["W"]] + list(fields["G"])
+        for a in vols:
+            _record(f, np.asarray(a, dtype=np.float64)[sj, si, :])
+    return path

@@ -26,7 +26,7 @@ class Params(ctypes.Structure):
         ("gmres_m", ctypes.c_int), ("reltol", ctypes.c_double), ("abstol", ctypes.c_double),
         ("maxit", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
         ("mg_maxgrids", ctypes.c_int), ("extrapolation", ctypes.c_int),
-        ("mg_mode", ctypes.c_int), ("cd_quirk", ctypes.c_int),
+        ("mg_mode", ctypes.c_int), ("cd_quirk", ctypes.c_int), ("gmres_side", ctypes.c_int),
     ]
 
 
@@ -61,6 +61,7 @@ def lib():
         L.owo_solve.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int), D]
         L.owo_vertical_velocity.argtypes = [P, D, D]
         L.owo_kinematics.argtypes = [P, D, D, D, D, D]
+        L.owo_kinematics_gradients.argtypes = [P, D, D, D]
         L.owo_rhs.argtypes = [P, D, D, D, D, D]
         L.owo_add_relax_zone.argtypes = [P, I, I, I, I, I, D]
         L.owo_rk4_step.restype = I
@@ -210,6 +211,15 @@ class OracleTank:
         lib().owo_kinematics(self.h, _p(phi), _p(E), _p(U), _p(V), _p(W))
         return U, V, W
 
+    def kinematics_gradients(self, phi, E):
+        """The 9 fields StoreKinematicData.f90:327-478 writes after u, v, w, shape (9,) + shape3: d w_z,
+        (d w)_x, (d w)_y, u_z, u_x, u_y, v_z, v_x, v_y (the first three from W = dphi/dsigma)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        G = np.zeros((9,) + self.shape3)
+        lib().owo_kinematics_gradients(self.h, _p(phi), _p(E), _p(G))
+        return G
+
     def rhs(self, E, P, W):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (E, P, W)]
         rE = np.zeros(self.shape2); rP = np.zeros(self.shape2)
@@ -329,6 +339,7 @@ def ref_lib():
         R.ref_transfer.argtypes = [I, I, I, I, D, D, D, I, I, I, D, D, D, D, D]
         R.ref_store_ep.argtypes = [D, D, I]
         R.ref_jonswap.argtypes = [I, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
+        R.ref_store_kinematics.argtypes = [D, IP, ctypes.c_double, I, I]
         _ref = R
     return _ref
 
@@ -390,6 +401,20 @@ class RefTank:
         rE = np.zeros(self.shape2); rP = np.zeros(self.shape2)
         ref_lib().ref_rhs(g, _p(W), _p(rE), _p(rP))
         return rE, rP
+
+    def store_kinematics(self, directory, phi, win, dt, it, nonlinear=1):
+        """StoreKinematicData's Kinematics_01_<it>.bin (formattype 21) for phi and the surface last given to
+        surface(), written into `directory`; win = (xbeg, xend, xstride, ybeg, yend, ystride, tbeg, tend,
+        tstride).  Returns the file path."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        w = np.ascontiguousarray(win, dtype=np.int32)
+        cwd = os.getcwd()
+        try:
+            os.chdir(directory)
+            ref_lib().ref_store_kinematics(_p(phi), _ip(w), float(dt), int(nonlinear), int(it))
+        finally:
+            os.chdir(cwd)
+        return os.path.join(directory, "Kinematics_01_%05d.bin" % it)
 
     def store_ep(self, directory, E, P, nr=0):
         """StoreData's EP_<nr>.bin for E, P, written into `directory`; returns the file path."""

@@ -321,6 +321,7 @@ struct owo_tank {
     long it_total, it_solves, it_min, it_max;
     /* solver workspace */
     double *kw;                              /* Krylov basis (m+1)*n */
+    double *kz;                              /* flexible GMRES: preconditioned basis vectors m*n */
     double *v1, *v2, *v3;
 };
 
@@ -529,6 +530,44 @@ void owo_kinematics(owo_tank *t, const double *phi, const double *E, double *U, 
             }
         }
     free(W);
+}
+
+/* The velocity gradients StoreKinematicData.f90:327-478 writes after u, v, w, as 9 fields of n in its order:
+ *   Dsigma(W)/d, Dx(W) + cx Dsigma(W), Dy(W) + cy Dsigma(W)      with W = dphi/dsigma (= d w)
+ *   Dsigma(U)/d, Dx(U) + cx Dsigma(U), Dy(U) + cy Dsigma(U)      (u_z, u_x, u_y)
+ *   Dsigma(V)/d, Dx(V) + cx Dsigma(V), Dy(V) + cy Dsigma(V)      (v_z, v_x, v_y)
+ * with U, V the velocities of owo_kinematics, cx = (1 - z)/d h_x - z/d eta_x and cy likewise (nonlinear mode
+ * only, :337-344, 353-360, ...), x (y) derivatives zero when Nx (Ny) is 1 (:346-348).  The first three are
+ * derivatives of d w, not of w: d w_z, (d w)_x, (d w)_y. */
+void owo_kinematics_gradients(owo_tank *t, const double *phi, const double *E, double *G) {
+    const grid_t *g = &t->g;
+    const size_t n = g->n;
+    double *U = xcalloc(n, sizeof(double)), *V = xcalloc(n, sizeof(double)), *Wv = xcalloc(n, sizeof(double));
+    double *W = xcalloc(n, sizeof(double)), *Fz = xcalloc(n, sizeof(double));
+    owo_kinematics(t, phi, E, U, V, Wv);
+    diff_z(g, phi, W, 1);
+    const double *F[3] = {W, U, V};
+    for (int f = 0; f < 3; f++) {
+        double *Gz = G + (size_t)(3 * f) * n, *Gx = Gz + n, *Gy = Gx + n;
+        diff_z(g, F[f], Fz, 1);
+        if (g->nx > 1) diff_x(g, g->nzg, F[f], Gx, 1); else memset(Gx, 0, n * sizeof(double));
+        if (g->ny > 1) diff_y(g, g->nzg, F[f], Gy, 1); else memset(Gy, 0, n * sizeof(double));
+        for (int j = 0; j < g->nyg; j++)
+            for (int i = 0; i < g->nxg; i++) {
+                const size_t p = I2(g, i, j);
+                const double d = g->h[p] + E[p];
+                for (int k = 0; k < g->nzg; k++) {
+                    const size_t q = I3(g, k, i, j);
+                    const double z = g->z[k];
+                    Gz[q] = Fz[q] / d;
+                    if (t->p.nonlinear) {
+                        if (g->nx > 1) Gx[q] = Gx[q] + ((1 - z) / d * g->hx[p] - z / d * t->Ex[p]) * Fz[q];
+                        if (g->ny > 1) Gy[q] = Gy[q] + ((1 - z) / d * g->hy[p] - z / d * t->Ey[p]) * Fz[q];
+                    }
+                }
+            }
+    }
+    free(U); free(V); free(Wv); free(W); free(Fz);
 }
 
 /* rhsFreeSurface3D.f90:1-29 and rhsLinearFreeSurface3D.f90:1-13 (no applied pressure,
@@ -1352,18 +1391,24 @@ static double dot(const double *a, const double *b, size_t n) {
  * SPARSKIT2's documented protocol): Arnoldi on M^-1 A with modified Gram-Schmidt plus
  * selective re-orthogonalisation, Givens rotations, stop when the estimated preconditioned
  * residual <= reltol*||M^-1 r0|| + abstol, at most m matrix-vector products in total (the
- * initial residual counts as one).  Returns 0 when converged, -1 when the product budget ran out. */
+ * initial residual counts as one).  Returns 0 when converged, -1 when the product budget ran out.
+ * gmres_side 1: flexible right preconditioning (FGMRES, Saad, SIAM J. Sci. Comput. 14, 1993): Arnoldi on
+ * A z_j with z_j = M^-1 v_j kept, x = x0 + Z y, stop on the estimated true residual <= reltol*||r0|| +
+ * abstol (no counterpart in the reference; the product uses it with an FP32 multigrid, which is linear
+ * only to FP32 rounding). */
 static int gmres_solve(owo_tank *t, const double *rhs, double *sol, int *iters, double *res) {
     size_t n = t->g.n;
     int m = t->p.gmres_m;
     if (m <= 1) m = 15;
-    double *V = t->kw, *w = t->v1, *z = t->v2;
+    const int flex = t->p.gmres_side == 1;
+    double *V = t->kw, *w = t->v1, *z = t->v2, *Z = t->kz;
     double *H = xcalloc((size_t)(m + 1) * (m + 1), sizeof(double));
     double *cs = xcalloc(m + 1, sizeof(double)), *sn = xcalloc(m + 1, sizeof(double)), *gv = xcalloc(m + 2, sizeof(double));
     int matvecs = 0, k = 0, status = -1;
     owo_matvec(t, sol, w); matvecs++;
     for (size_t i = 0; i < n; i++) w[i] = rhs[i] - w[i];
-    owo_precond(t, w, V);
+    if (flex) memcpy(V, w, n * sizeof(double));
+    else owo_precond(t, w, V);
     double beta = sqrt(dot(V, V, n));
     double tol = t->p.reltol * beta + t->p.abstol;
     *res = beta;
@@ -1374,8 +1419,14 @@ static int gmres_solve(owo_tank *t, const double *rhs, double *sol, int *iters, 
     while (1) {
         k++;
         double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
-        owo_matvec(t, vk, w); matvecs++;
-        owo_precond(t, w, vn);
+        if (flex) {
+            double *zk = Z + (size_t)(k - 1) * n;
+            owo_precond(t, vk, zk);
+            owo_matvec(t, zk, vn); matvecs++;
+        } else {
+            owo_matvec(t, vk, w); matvecs++;
+            owo_precond(t, w, vn);
+        }
         /* MGS with selective re-orthogonalisation (cosine > 0.99) */
         double nrm0 = dot(vn, vn, n), thr = nrm0 * 0.99 * 0.99;
         for (int i = 0; i < k; i++) {
@@ -1423,7 +1474,8 @@ static int gmres_solve(owo_tank *t, const double *rhs, double *sol, int *iters, 
             for (int q = i + 1; q < kk; q++) s -= H[(size_t)i * m + q] * y[q];
             y[i] = s / H[(size_t)i * m + i];
         }
-        for (int i = 0; i < kk; i++) { double *vi = V + (size_t)i * n; for (size_t q = 0; q < n; q++) sol[q] += y[i] * vi[q]; }
+        const double *B = flex ? Z : V;
+        for (int i = 0; i < kk; i++) { const double *vi = B + (size_t)i * n; for (size_t q = 0; q < n; q++) sol[q] += y[i] * vi[q]; }
         free(y);
     }
     status = (est <= tol) ? 0 : -1;
@@ -1574,6 +1626,7 @@ owo_tank *owo_create(const owo_params *p, const double *h, const double *hx, con
     for (int q = 0; q < 10; q++) t->tmp[q] = xcalloc(n, 8);
     int m = p->gmres_m > 1 ? p->gmres_m : 15;
     t->kw = xcalloc((size_t)(m + 1) * n, 8);
+    if (t->p.gmres_side == 1) t->kz = xcalloc((size_t)m * n, 8);
     t->v1 = xcalloc(n, 8); t->v2 = xcalloc(n, 8); t->v3 = xcalloc(n, 8);
     t->it_min = -1;
     if (p->precond == 3 && mg_setup(t) != 0) { owo_destroy(t); return NULL; }
@@ -1586,7 +1639,7 @@ void owo_destroy(owo_tank *t) {
     free(t->ds); free(t->E); free(t->P); free(t->W); free(t->Ex); free(t->Exx); free(t->Ey); free(t->Eyy);
     free(t->Px); free(t->Py); free(t->phi); free(t->rhs); free(t->last[0]); free(t->last[1]);
     for (int q = 0; q < 10; q++) free(t->tmp[q]);
-    free(t->kw); free(t->v1); free(t->v2); free(t->v3);
+    free(t->kw); free(t->kz); free(t->v1); free(t->v2); free(t->v3);
     for (int k = 0; k < t->nlev; k++) {
         level_t *L = &t->lev[k];
         grid_free(&L->g); free(L->ds); free(L->rp); free(L->ci); free(L->va); free(L->ighost);

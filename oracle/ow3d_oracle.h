@@ -33,6 +33,8 @@ typedef struct {
     int mg_mode;               /* 0: reference CSR + lexicographic Gauss-Seidel, 1: matrix-free zebra z-line */
     int cd_quirk;              /* mg_mode 0 only: 1 = replicate the reference's cross-derivative
                                   stencil over the transverse window (ConstructTableCrossDerivatives.f90:312-347) */
+    int gmres_side;            /* 0: left-preconditioned GMRES (the reference's), 1: flexible
+                                  right-preconditioned GMRES (FGMRES, Saad 1993) */
 } owo_params;
 
 typedef struct owo_tank owo_tank;
@@ -77,6 +79,8 @@ int  owo_level_grid(const owo_tank *t, int level, int *dims6, double *x, double 
                     double *hx, double *hy, double *hxx, double *hyy);
 int  owo_level_ghosts(const owo_tank *t, int level, int *out);
 void owo_gs_once(owo_tank *t, int level, const double *b, double *x);
+/* The 9 velocity-gradient fields StoreKinematicData.f90:327-478 writes after u, v, w (G: 9 n). */
+void owo_kinematics_gradients(owo_tank *t, const double *phi, const double *E, double *G);
 /* One pass of the zebra-line smoother on a level: colour 0 / 1 relaxes the interior sigma-columns with
  * (i + j) even / odd, colour 2 is the wall-ghost relaxation pass. */
 void owo_zebra(owo_tank *t, int level, const double *b, double *x, int colour);

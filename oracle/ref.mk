@@ -19,7 +19,7 @@ REF_SRCS := functions/BuildLinearSystem.f90 functions/DiffXEven.f90 functions/Di
             timeintegration/DifferentiationsFreeSurfacePlane.f90 main/rhsFreeSurface3D.f90 \
             multigrid/GaussSeidel.f90 multigrid/GaussSeidelSpecialPoints.f90 \
             multigrid/MGRestriction3DExplicit.f90 multigrid/MGProlongation3DExplicit.f90 IO/CheckError.f90 \
-            IO/StoreData.f90 utilities/JONSWAP_spectrum.f90
+            IO/StoreData.f90 utilities/JONSWAP_spectrum.f90 IO/StoreKinematicData.f90
 
 ref: _ref/libow3d_ref.so
 

@@ -316,4 +316,33 @@ CONTAINS
     END DO
   END SUBROUTINE ref_jonswap
 
+  ! StoreKinematicData(Nx, Ny, Nz, io, it) (IO/StoreKinematicData.f90) with formattype 21 for one output
+  ! window: writes Kinematics_01_<it>.bin into the current directory from phi_in and the surface last given
+  ! to ref_surface.  win = xbeg, xend, xstride, ybeg, yend, ystride, tbeg, tend, tstride.
+  SUBROUTINE ref_store_kinematics(phi_in, win, dt_, nonlin, it) BIND(C, name="ref_store_kinematics")
+    USE GlobalVariables, ONLY: PHI, Output, formattype, dt, LinearONOFF
+    REAL(C_DOUBLE), DIMENSION(*) :: phi_in
+    INTEGER(C_INT), DIMENSION(9) :: win
+    REAL(C_DOUBLE), VALUE :: dt_
+    INTEGER(C_INT), VALUE :: nonlin, it
+    INTEGER :: nxg, nyg, nzg, i, j, k
+    nxg = FineGrid%Nx + 2*GhostGridX; nyg = FineGrid%Ny + 2*GhostGridY; nzg = FineGrid%Nz + GhostGridZ
+    IF (ASSOCIATED(PHI)) DEALLOCATE(PHI)
+    ALLOCATE(PHI(nzg, nxg, nyg))
+    DO j = 1, nyg
+       DO i = 1, nxg
+          DO k = 1, nzg
+             PHI(k, i, j) = phi_in(k + nzg*(i-1) + nzg*nxg*(j-1))
+          END DO
+       END DO
+    END DO
+    IF (ALLOCATED(Output)) DEALLOCATE(Output)
+    ALLOCATE(Output(1))
+    Output(1)%xbeg = win(1); Output(1)%xend = win(2); Output(1)%xstride = win(3)
+    Output(1)%ybeg = win(4); Output(1)%yend = win(5); Output(1)%ystride = win(6)
+    Output(1)%tbeg = win(7); Output(1)%tend = win(8); Output(1)%tstride = win(9)
+    formattype = 21; dt = dt_; LinearONOFF = nonlin
+    CALL StoreKinematicData(nxg, nyg, nzg, 1, INT(it))
+  END SUBROUTINE ref_store_kinematics
+
 END MODULE ow3d_ref_driver

@@ -36,7 +36,7 @@ PROGRAM boundary_check
   c%nonlinear = 1; c%solver = 1; c%precond = 3; c%gmres_m = 23
   c%reltol = 1.0e-10_C_DOUBLE; c%abstol = 0.0_C_DOUBLE
   c%maxit = 1; c%nu1 = 1; c%nu2 = 1; c%mg_maxgrids = 20; c%extrapolation = 1; c%mg_precision = 0
-  c%pressure_term = 0; c%pressure_damping = 0
+  c%pressure_term = 0; c%pressure_damping = 0; c%gmres_side = 0
 
   rc = ow3d_create(c, h, hx, hy, hxx, hyy, tank)
   IF (rc /= 0) STOP 'ow3d_create failed'

@@ -12,6 +12,7 @@ outputs for them:
   rhsE, rhsP                         rhsFreeSurface3D.f90
   restriction / prolongation         MGRestriction3DExplicit.f90, MGProlongation3DExplicit.f90
   JONSWAP spectrum (jonswap_ref.npz) JONSWAP_spectrum.f90:1-30
+  kinematics files (kinematics_ref.npz)  StoreKinematicData.f90:90-478 (formattype 21), inputs and the bytes
 
 Usage: python tests/golden/make_golden.py   (writes tests/golden/ref_<case>.npz)
 """
@@ -92,6 +93,26 @@ def main():
         out.update({f"f{q}": f, f"S{q}": oracle.ref_jonswap(f, hs, tp, gam), f"par{q}": np.array([hs, tp, gam])})
     np.savez_compressed(os.path.join(HERE, "jonswap_ref.npz"), **out)
     print("wrote jonswap_ref")
+    # kinematics files of an 11 x 9 x 6 shoal tank (4th order) for a window with an x stride of 2
+    import tempfile
+    nx, ny, nz, alpha, L = 11, 9, 6, 2, 6.0
+    bath = problems.gaussian_shoal(nx, ny, L, L)
+    x, y = problems.coords(nx, ny, L, L)
+    X, Y = np.meshgrid(x, y)
+    rng = np.random.default_rng(17)
+    E = 0.05 * np.cos(2 * np.pi * X / L) * np.cos(2 * np.pi * Y / L) + 0.002 * rng.standard_normal(X.shape)
+    P = 0.1 * rng.standard_normal(X.shape)
+    phi = rng.standard_normal((ny + 2, nx + 2, nz + 1))
+    win = np.array([2, nx - 1, 2, 1, ny - 2, 1, 1, 10, 1], dtype=np.int32)
+    R = oracle.RefTank(nx, ny, nz, L, L, alpha, bath=bath)
+    R.surface(E, P)
+    with tempfile.TemporaryDirectory() as d:
+        f0 = open(R.store_kinematics(d, phi, win, 0.05, 0), "rb").read()
+        f1 = open(R.store_kinematics(d, phi, win, 0.05, 1), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "kinematics_ref.npz"), nx=nx, ny=ny, nz=nz, alpha=alpha, L=L, win=win,
+                        dt=0.05, E_in=E, P_in=P, phi=phi, **{k: bath[k] for k in bath},
+                        file0=np.frombuffer(f0, dtype=np.uint8), file1=np.frombuffer(f1, dtype=np.uint8))
+    print("wrote kinematics_ref")
 
 
 if __name__ == "__main__":

@@ -52,7 +52,7 @@ def test_config_layout_matches_header():
 
 
 @pytest.mark.parametrize("bad", [dict(alpha=3, beta=2), dict(gmres_m=40), dict(mg_precision=5), dict(nz=2),
-                                 dict(precond=1), dict(pressure_term=1), dict(pressure_damping=1)])
+                                 dict(precond=1), dict(pressure_term=1), dict(pressure_damping=1), dict(gmres_side=2)])
 def test_invalid_configuration_is_rejected_on_the_host(bad):
     import ow3d
     cfg = dict(nx=17, ny=17, nz=9)

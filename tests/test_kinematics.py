@@ -114,3 +114,73 @@ def test_gpu_kinematics_match_the_oracle(oracle_mod, nx, ny, nz, order):
         assert np.max(np.abs(g - o)) <= 1e-9 * max(np.max(np.abs(o)), 1e-300)
     # an explicit phi gives the same as the resident one
     assert np.array_equal(T_g.kinematics(phig)[0], T_g.kinematics()[0])
+
+
+def _exact_gradients(nx, ny, nz, L=6.0):
+    """The 9 fields StoreKinematicData.f90:327-478 writes after u, v, w, exact for the potential of _exact:
+    d w_z, d w_x + d_x w, d w_y + d_y w (derivatives of W = dphi/dsigma = d w), u_z, u_x, u_y, v_z, v_x, v_y."""
+    from ow3d import problems
+    bath, eta, phi, (u, v, w) = _exact(nx, ny, nz, 6, L)
+    x, y = problems.coords(nx, ny, L, L)
+    X, Y = np.meshgrid(x, y)
+    k1, k2 = 2 * np.pi / L, 4 * np.pi / L
+    kap = np.hypot(k1, k2)
+    etax = -0.08 * k1 * np.sin(k1 * X) * np.cos(k1 * Y)
+    etay = -0.08 * k1 * np.cos(k1 * X) * np.sin(k1 * Y)
+    zs = np.sin(np.pi * np.arange(nz) / (2 * (nz - 1)))
+    sig = np.concatenate([[-zs[1]], zs])
+    d = (bath["h"] + eta)[:, :, None]
+    Z = sig[None, None, :] * d - bath["h"][:, :, None]
+    C, S = np.cos(k1 * X)[:, :, None], np.sin(k1 * X)[:, :, None]
+    Cy, Sy = np.cos(k2 * Y)[:, :, None], np.sin(k2 * Y)[:, :, None]
+    ch, sh = np.cosh(kap * (Z + 1.3)), np.sinh(kap * (Z + 1.3))
+    dx, dy = (bath["hx"] + etax)[:, :, None], (bath["hy"] + etay)[:, :, None]
+    G = [d * kap ** 2 * ch * C * Cy, d * (-k1 * kap * sh * S * Cy) + dx * w, d * (-k2 * kap * sh * C * Sy) + dy * w,
+         -k1 * kap * sh * S * Cy, -k1 ** 2 * ch * C * Cy, k1 * k2 * ch * S * Sy,
+         -k2 * kap * sh * C * Sy, k1 * k2 * ch * S * Sy, -k2 ** 2 * ch * C * Cy]
+    return bath, eta, phi, G
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_oracle_kinematics_gradients_converge_to_the_exact_values(oracle_mod, order):
+    """All nine gradient fields converge under refinement at close to the stencil order (second derivatives
+    composed of first-derivative passes, so one order below the velocities); a dropped chain-rule term, a
+    missing 1/d, or gradients of w instead of W = d w would leave an O(1) error."""
+    a = order // 2
+    errs = []
+    for (n, nz) in ((17, 9), (33, 17), (65, 33)):
+        bath, eta, phi, G = _exact_gradients(n, n, nz)
+        p = oracle_params(n, n, nz, alpha=a, beta=a, gamma=a, lx=6.0, ly=6.0, precond=0)
+        T = oracle_mod.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+        E, _, _ = T.surface(eta, np.zeros_like(eta))
+        got = T.kinematics_gradients(phi, E)
+        errs.append([float(np.max(np.abs(got[q][2:-2, 2:-2, 1:] - G[q][2:-2, 2:-2, 1:]))) for q in range(9)])
+    errs = np.array(errs)
+    rates = np.log2(errs[:-1] / errs[1:])
+    assert np.all(errs[-1] < {4: 2e-2, 6: 6e-4}[order]), errs[-1]
+    assert np.all(rates[-1] > order - 2.5), rates
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny,nz,order", [(33, 33, 9, 6), (65, 1, 9, 6), (33, 17, 12, 8)])
+def test_gpu_kinematics_gradients_match_the_oracle(oracle_mod, nx, ny, nz, order):
+    import ow3d
+    from ow3d import problems
+    a = order // 2
+    L = 10.0
+    bath = problems.gaussian_shoal(nx, ny, L, L)
+    x, y = problems.coords(nx, ny, L, L)
+    X, Y = np.meshgrid(x, y)
+    E = 0.02 * np.sin(2 * np.pi * X / L) * np.cos(2 * np.pi * Y / L)
+    P = 0.01 * np.cos(2 * np.pi * X / L)
+    p = oracle_params(nx, ny, nz, alpha=a, beta=a, gamma=a, lx=L, ly=L, mg_maxgrids=0)
+    T_o = oracle_mod.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
+    T_g = ow3d.Tank({k: v for k, v in p.items() if k not in ("mg_mode", "cd_quirk")}, **bath)
+    phi = np.random.default_rng(5).standard_normal(T_g.shape3) * 0.01
+    T_g.set_state(E, P, None, phi)
+    T_o.set_state(E, P, None, phi)
+    E_o, _, _, _ = T_o.get_state()
+    Gg, Go = T_g.kinematics_gradients(), T_o.kinematics_gradients(phi, E_o)
+    for q in range(9):
+        assert np.max(np.abs(Gg[q] - Go[q])) <= 1e-12 * max(np.max(np.abs(Go[q])), 1e-300), q
+    assert np.array_equal(T_g.kinematics_gradients(phi), Gg)

@@ -45,6 +45,8 @@ def test_device_memory_estimate():
     ng = 131 * 131 * 10
     assert e32 < e64 and e15 < e32
     assert e32 - e15 == 8 * 8 * ng              # eight fewer Krylov basis vectors
+    # flexible GMRES keeps m preconditioned vectors more
+    assert ow3d.estimate_bytes(dict(base, mg_precision=1, gmres_side=1)) - e32 == 23 * 8 * ng
     # fine-grid vectors + basis (m = 23) + step snapshot dominate: about (8 + 24 + 1) vectors
     assert 33 * 8 * ng < e64 < 45 * 8 * ng
     with pytest.raises(ow3d.OW3DError):

@@ -129,13 +129,15 @@ def test_vcycle_contracts(mode):
     assert rate < (0.6 if mode == 0 else 0.35), rate
 
 
-@pytest.mark.parametrize("solver", [1, 0])
-def test_krylov_converges_to_dense_solution(solver):
+@pytest.mark.parametrize("solver,side", [(1, 0), (1, 1), (0, 0)])
+def test_krylov_converges_to_dense_solution(solver, side):
+    """GMRES with left preconditioning (the reference's) and flexible right preconditioning, and defect
+    correction, against numpy's dense solve of the assembled high-order system."""
     pr = P()
     nx, ny, nz = 9, 9, 5
     bath = pr.gaussian_shoal(nx, ny, 4.0, 4.0)
     p = oracle_params(nx, ny, nz, alpha=2, beta=2, gamma=2, lx=4.0, ly=4.0, solver=solver, reltol=1e-14,
-                      gmres_m=31 if solver else 31)
+                      gmres_m=31, gmres_side=side)
     T = oracle.OracleTank(p, bath["h"], bath["hx"], bath["hy"], bath["hxx"], bath["hyy"])
     n = T.n
     A = np.zeros((n, n))

@@ -8,12 +8,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _tank(n, nz, solver):
+def _tank(n, nz, solver, prec=0):
     import ow3d
     from ow3d import problems
     L = 10.0
     bath = problems.gaussian_shoal(n, n, L, L)
-    T = ow3d.Tank(dict(nx=n, ny=n, nz=nz, lx=L, ly=L, solver=solver, reltol=1e-10, gmres_m=23), **bath)
+    T = ow3d.Tank(dict(nx=n, ny=n, nz=nz, lx=L, ly=L, solver=solver, reltol=1e-10, gmres_m=23, mg_precision=prec,
+                       gmres_side=prec), **bath)
     x, y = problems.coords(n, n, L, L)
     X, Y = np.meshgrid(x, y)
     E = 0.02 * np.sin(2 * np.pi * X / L + 0.3) * np.cos(2 * np.pi * Y / L)
@@ -25,11 +26,13 @@ def _tank(n, nz, solver):
     return T, b, x0
 
 
-@pytest.mark.parametrize("solver", [1, 0])
-def test_noalloc_solve_equals_solve(solver):
+@pytest.mark.parametrize("solver,prec", [(1, 0), (0, 0), (1, 1)])
+def test_noalloc_solve_equals_solve(solver, prec):
+    """GMRES (left preconditioning with the FP64 multigrid; flexible with the FP32 one, whose workspace
+    also holds the preconditioned basis) and defect correction."""
     for n, nz in ((65, 9), (33, 12)):   # second call on a smaller tank
-        print(f"no-alloc solve: {n}x{n}x{nz}, solver {solver}", flush=True)
-        T, b, x0 = _tank(n, nz, solver)
+        print(f"no-alloc solve: {n}x{n}x{nz}, solver {solver}, multigrid precision {prec}", flush=True)
+        T, b, x0 = _tank(n, nz, solver, prec)
         got = T.solve_noalloc(b, x0)
         want = T.solve(b, x0)
         assert np.array_equal(got[0], want[0]), (n, nz)
