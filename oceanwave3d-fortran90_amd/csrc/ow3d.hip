@@ -45,6 +45,8 @@ int red_blocks(size_t n) { return (int)std::min<size_t>(RED_BLOCKS, std::max<siz
 // Krylov kernel dispatch on the number of basis vectors (1 .. MAXK-1).
 #define OW3D_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
     X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+#define OW3D_K_CASES_2(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
 static_assert(MAXK == 32, "Krylov dispatch covers 1..31 vectors");
 
 int launch_mdot(int k, bool norm, int nb, size_t n, const double *V, const double *w, double *part, const int *skip, hipStream_t s) {
@@ -67,6 +69,25 @@ int launch_gs_update_dot(int k, int nb, size_t n, const double *V, const double 
     err() = "Krylov dimension out of range";
     return -1;
 }
+int launch_dcgs_a(int k, int nb, size_t n, double *V, const KrylovState *S, double *part, const int *skip, hipStream_t s) {
+    switch (k) {
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_dcgs_a<K, MAXK>), dim3(nb), dim3(256), 0, s, n, V, S, part, skip); return 0;
+        OW3D_K_CASES_2(OW3D_C)
+#undef OW3D_C
+    }
+    err() = "Krylov dimension out of range";
+    return -1;
+}
+int launch_dcgs_b(int k, int nb, size_t n, const double *V, const double *h, const KrylovState *S, double *v, double *part,
+                  const int *skip, hipStream_t s) {
+    switch (k) {
+#define OW3D_C(K) case K: hipLaunchKernelGGL((k_dcgs_b<K, MAXK>), dim3(nb), dim3(256), 0, s, n, V, h, S, v, part, skip); return 0;
+        OW3D_K_CASES(OW3D_C)
+#undef OW3D_C
+    }
+    err() = "Krylov dimension out of range";
+    return -1;
+}
 int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, const double *scale, double *v, const int *skip,
                      hipStream_t s) {
     switch (k) {
@@ -79,24 +100,28 @@ int launch_gs_update(int k, int nb, size_t n, const double *V, const double *h, 
 }
 
 // Workspace of one Laplace solve: the Krylov basis V (m + 1 vectors of n), the residual w and correction z
-// vectors, the partial sums of the reductions (RED_BLOCKS x MAXK) and their totals hv (MAXK).
+// vectors, the partial sums of the reductions (RED_BLOCKS x MAXK) and their totals hv (2 x MAXK: the
+// two reductions of a step).
 struct SolveWork {
     double *V, *w, *z, *part, *hv;
     double *Z;   // flexible GMRES: z_j = M^-1 v_j (m vectors of n)
 };
 SolveWork handle_work(ow3d_tank *t) { return {t->V.p, t->w.p, t->z.p, t->part.p, t->hv.p, t->Zf.p}; }
 
-// hv[0..k-1] = sum over the partial sums of the last reduction
-void reduce(const SolveWork &W, int k, int nb, const int *skip, hipStream_t s) {
-    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)W.part, W.hv, skip);
+// hv[0..k-1] (with second: hv[MAXK..MAXK+k-1]) = sum over the partial sums of the last reduction
+void reduce(const SolveWork &W, int k, int nb, const int *skip, hipStream_t s, bool second = false) {
+    hipLaunchKernelGGL((k_reduce_parts<MAXK>), dim3(k), dim3(256), 0, s, nb, k, (const double *)W.part, W.hv + (second ? MAXK : 0), skip);
 }
 
-// Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres): classical
-// Gram-Schmidt with the DGKS second pass when the projection removed more than half of the squared norm.
-// At most m operator applications (the initial residual and m - 1 Arnoldi steps), as the reference's
-// ipar(5) = ipar(6) = m setting.  With t->flexible, right preconditioning with the preconditioned vectors
-// kept (FGMRES, Saad 1993): the basis spans A z_j with z_j = M^-1 v_j, x = x0 + Z y, and the stopping test
-// is on the true residual, whose attainable accuracy does not depend on how exactly M^-1 is linear.
+// Left-preconditioned GMRES(m), one cycle (iterative_solution.f90 with SPARSKIT2 gmres), orthogonalised
+// by one-reduction classical Gram-Schmidt with reorthogonalisation (k_dcgs_a / k_dcgs_b / k_dcgs_column:
+// every step projects twice, in two reads of the basis).  At most m operator applications (the initial
+// residual and m - 1 Arnoldi steps), as the reference's ipar(5) = ipar(6) = m setting.  With t->flexible,
+// right preconditioning with the preconditioned vectors kept (FGMRES, Saad 1993): the basis spans A z_j
+// with z_j = M^-1 v_j, x = x0 + Z y, and the stopping test is on the true residual, whose attainable
+// accuracy does not depend on how exactly M^-1 is linear; the preconditioner must then see each basis
+// vector itself, so that path uses classical Gram-Schmidt with the DGKS second pass when the projection
+// removed more than half of the squared norm.
 int full_slots(const ow3d_tank *t) {
     if (t->cfg.solver == 0) return std::max(1, t->cfg.gmres_m);
     const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
@@ -127,13 +152,21 @@ int gmres(ow3d_tank *t, const SolveWork &W, const double *b, double *x, hipStrea
     hipLaunchKernelGGL(k_scale_state, dim3(nbl), dim3(256), 0, s, n, (const KrylovState *)S, V);
     for (int k = 1; k <= slots; k++) {
         double *vk = V + (size_t)(k - 1) * n, *vn = V + (size_t)k * n;
-        if (flex) {   // z = M^-1 v_k-1, kept; the new direction is A z
+        if (!flex) {   // one-reduction CGS2: the operator is applied to the pending candidate in slot k - 1
+            if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
+            if (precond(t, w, vn, s)) return -1;
+            if (k == 1) { if (launch_mdot(1, false, nb, n, V, vn, W.part, done, s)) return -1; }
+            else if (launch_dcgs_a(k, nb, n, V, S, W.part, done, s)) return -1;
+            reduce(W, k, nb, done, s);
+            if (launch_dcgs_b(k, nb, n, V, W.hv, S, vn, W.part, done, s)) return -1;
+            reduce(W, k + 1, nb, done, s, true);
+            hipLaunchKernelGGL(k_dcgs_column, dim3(1), dim3(64), 0, s, S, (const double *)W.hv, (const double *)(W.hv + MAXK), k, m);
+            continue;
+        }
+        {   // z = M^-1 v_k-1, kept; the new direction is A z
             double *zk = W.Z + (size_t)(k - 1) * n;
             if (precond(t, vk, zk, s)) return -1;
             if (launch_matvec(t, zk, vn, s, done)) { err() = "matvec dispatch failed"; return -1; }
-        } else {
-            if (launch_matvec(t, vk, w, s, done)) { err() = "matvec dispatch failed"; return -1; }
-            if (precond(t, w, vn, s)) return -1;
         }
         if (launch_mdot(k, true, nb, n, V, vn, W.part, done, s)) return -1;
         reduce(W, k + 1, nb, done, s);
@@ -354,7 +387,7 @@ extern "C" int ow3d_create(const ow3d_config *cfg, const double *h, const double
     }
     const int m = c.gmres_m > 1 ? c.gmres_m : 15;
     HIPCHK(t->part.alloc((size_t)RED_BLOCKS * MAXK));
-    HIPCHK(t->hv.alloc(MAXK));
+    HIPCHK(t->hv.alloc(2 * MAXK));
     HIPCHK(t->V.alloc((size_t)(m + 1) * n));
     HIPCHK(hipMemset(t->V.p, 0, (size_t)(m + 1) * n * 8));   // unused basis vectors stay finite
     t->flexible = c.solver == 1 && c.gmres_side == 1;
@@ -484,7 +517,7 @@ extern "C" int ow3d_solve_noalloc(ow3d_tank *t, const double *rhs, double *sol, 
     if (t->async_busy) { err() = "asynchronous steps still in flight"; return -1; }
     const size_t n = t->g.n, np = pad32(n);
     const int m = t->cfg.gmres_m > 1 ? t->cfg.gmres_m : 15;
-    const size_t head = pad32((size_t)RED_BLOCKS * MAXK) + pad32(MAXK);
+    const size_t head = pad32((size_t)RED_BLOCKS * MAXK) + pad32(2 * MAXK);
     const size_t nbasis = (size_t)(m + 1) + (t->flexible ? (size_t)m : 0);   // V, then Z
     const size_t need = head + (4 + nbasis) * np;
     if (need > SCRATCH_DOUBLES) {
@@ -984,7 +1017,7 @@ extern "C" long long ow3d_estimate_bytes(const ow3d_config *cfg) {
     const bool flex = cfg->solver == 1 && cfg->gmres_side == 1;
     if (flex) b += 8 * n * m;                                        // preconditioned basis of flexible GMRES
     b += 8LL * ((long long)g.nxg * 2 * g.ra + (cfg->ny > 1 ? (long long)g.nyg * 2 * g.rb : 6) + 2LL * g.nzg * 2 * g.rc);
-    b += 8LL * RED_BLOCKS * MAXK + 8LL * MAXK + (long long)sizeof(KrylovState);
+    b += 8LL * RED_BLOCKS * MAXK + 16LL * MAXK + (long long)sizeof(KrylovState);
     if (cfg->precond == 3) {
         std::vector<Grid> lev;
         if (!mg_levels(g, cfg->mg_maxgrids, lev, err())) return -1;

@@ -1202,6 +1202,11 @@ struct KrylovState {
     int overflow, amax[4];
     long long sv_total, sv_solves;   // statistics at the start of the step (restored when it is replayed)
     int sv_min, sv_max, sv_status;
+    // one-reduction CGS2 (left preconditioning): the Hessenberg columns before rotation, and the pending
+    // basis vector's second-pass projections cv and inverse norm ib (the vector is normalised in the next
+    // step's first pass)
+    double Hr[KRYLOV_MAXK * KRYLOV_MAXK];
+    double cv[KRYLOV_MAXK], ib;
 };
 
 // Start of a speculative RK4 step: clear the per-step report, keep the statistics for a replay.
@@ -1239,6 +1244,7 @@ static __global__ void k_gmres_start(KrylovState *S, const double *hv, double rt
     S->beta = beta; S->tol = rtol * beta + atol; S->est = beta;
     if (beta <= S->tol) { S->done = 1; S->status = 0; return; }
     S->scale = 1.0 / beta;
+    S->ib = 1.0;
     S->g[0] = beta;
 }
 
@@ -1317,6 +1323,107 @@ static __global__ __launch_bounds__(64) void k_gmres_pass2(KrylovState *S, const
         col[k] = sqrt(fmax(n2, 0.0));
         gmres_column(S, k, col, cs, sn, m);
         S->skip2 = 1;
+    }
+    __syncthreads();
+    if (t <= k) S->H[t * M + (k - 1)] = col[t];
+}
+
+// One-reduction classical Gram-Schmidt with reorthogonalisation (DCGS2: Swirydowicz, Langou, Ananthan,
+// Yang and Thomas, Numer. Linear Algebra Appl. 28 (2021) e2343; Bielich et al., Parallel Comput. 112
+// (2022) 102940) for left-preconditioned GMRES.  Step k (1-based) applies the operator to the candidate u
+// in slot k-1 -- the once-orthogonalised, unnormalised vector of step k-1 -- rather than to the basis
+// vector V_{k-1} = (u - sum_{i<k-1} c_i V_i) ib, whose second-pass projections c and inverse norm ib were
+// found in step k-1.  By linearity z = M^-1 A u = (1/ib) M^-1 A V_{k-1} + V H c (H the unrotated
+// Hessenberg columns 0..k-2), so with d = V^T z over V_0..V_{k-1}:
+//   first pass: V_{k-1} normalised in place and d in one read of the basis (k_dcgs_a),
+//   second pass: u' = (z - V d) ib and its projections c' = V^T u', |u'|^2 in one more read (k_dcgs_b),
+//   column k-1: h = (d - H c) ib, H(0..k-1, k-1) = h + c', H(k, k-1) = sqrt(|u'|^2 - |c'|^2).
+// Two reads of the basis per step instead of three (projection, second projection, update).
+
+// Step k >= 2, first pass: slot K-1 holds u; V_{K-1} = (u - sum_{i<K-1} c_i V_i) ib is written back, and
+// part gets the dots of z (slot K) with V_0 .. V_{K-1}.
+template <int K, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_dcgs_a(size_t n, double *__restrict__ V, const KrylovState *S,
+                                                 double *__restrict__ part, const int *skip) {
+    static_assert(K >= 2, "the first step has no pending vector");
+    if (solve_done(skip)) return;
+    double c[K - 1], acc[K];
+#pragma unroll
+    for (int i = 0; i < K - 1; i++) c[i] = S->cv[i];
+    const double ib = S->ib;
+#pragma unroll
+    for (int i = 0; i < K; i++) acc[i] = 0.0;
+    double *__restrict__ u = V + (size_t)(K - 1) * n;
+    const double *__restrict__ z = V + (size_t)K * n;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double vv[K - 1];
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) vv[i] = V[(size_t)i * n + q];
+        double x = u[q];
+        const double zq = z[q];
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) x -= c[i] * vv[i];
+        x *= ib;
+        u[q] = x;
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) acc[i] += vv[i] * zq;
+        acc[K - 1] += x * zq;
+    }
+    block_reduce_store<K, PSTRIDE>(acc, part);
+}
+
+// Step k, second pass: v <- (v - sum_{i<K} h_i V_i) ib (h the first pass's dots); part gets the dots of the
+// new v with V_i (i < K) and its squared norm (index K).
+template <int K, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_dcgs_b(size_t n, const double *__restrict__ V, const double *__restrict__ h,
+                                                 const KrylovState *S, double *__restrict__ v, double *__restrict__ part,
+                                                 const int *skip) {
+    if (solve_done(skip)) return;
+    double acc[K + 1], hh[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) { acc[i] = 0.0; hh[i] = h[i]; }
+    acc[K] = 0.0;
+    const double ib = S->ib;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        double vv[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) vv[i] = V[(size_t)i * n + q];
+        double x = v[q];
+#pragma unroll
+        for (int i = 0; i < K; i++) x -= hh[i] * vv[i];
+        x *= ib;
+        v[q] = x;
+#pragma unroll
+        for (int i = 0; i < K; i++) acc[i] += vv[i] * x;
+        acc[K] += x * x;
+    }
+    block_reduce_store<K + 1, PSTRIDE>(acc, part);
+}
+
+// Step k: d = the first pass's dots (k), e = the second pass's dots and squared norm (k + 1).  Completes
+// Hessenberg column k - 1, rotates it, updates the residual estimate and the stopping test, and keeps c'
+// and the new inverse norm for the next step.
+static __global__ __launch_bounds__(64) void k_dcgs_column(KrylovState *S, const double *d, const double *e, int k, int m) {
+    constexpr int M = KRYLOV_MAXK;
+    __shared__ double col[M + 1], cs[M], sn[M];
+    if (S->done) return;
+    const int t = threadIdx.x;
+    if (t < k) {   // h_t = (d_t - sum_j H(t, j) c_j) ib over the columns j = t-1 .. k-2 that have row t
+        double acc = d[t];
+        for (int j = t > 0 ? t - 1 : 0; j <= k - 2; j++) acc -= S->Hr[t * M + j] * S->cv[j];
+        col[t] = acc * S->ib + e[t];
+    }
+    if (t < k - 1) { cs[t] = S->cs[t]; sn[t] = S->sn[t]; }
+    __syncthreads();
+    if (t == 0) {
+        S->matvecs += 1;
+        double n2 = e[k];
+        for (int i = 0; i < k; i++) n2 -= e[i] * e[i];
+        col[k] = sqrt(fmax(n2, 0.0));
+        for (int i = 0; i <= k; i++) S->Hr[i * M + (k - 1)] = col[i];
+        for (int i = 0; i < k; i++) S->cv[i] = e[i];
+        S->ib = col[k] > 0.0 ? 1.0 / col[k] : 1.0;
+        gmres_column(S, k, col, cs, sn, m);
     }
     __syncthreads();
     if (t <= k) S->H[t * M + (k - 1)] = col[t];
