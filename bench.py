@@ -153,6 +153,10 @@ def main():
     ap.add_argument("--untuned", action="store_true", help="--batch / --serve without the autotuner's cached schedules")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
                     help="request-loop workload instead: Poisson arrivals served for SECONDS (prints its own JSON line)")
+    ap.add_argument("--profile-steps", type=int, default=0, metavar="K",
+                    help="only run 2 + K RK4 steps of the 513^2 workload with the multigrid precision of "
+                         "--precision, then exit (for rocprofv3 kernel traces)")
+    ap.add_argument("--precision", type=int, default=1, help="with --profile-steps: 0 FP64, 1 FP32, 2 FP16 coarse")
     ap.add_argument("--profile-matvec", type=int, default=0, metavar="K",
                     help="only run K high-order matvec launches of the workload and a 256 MiB device copy "
                          "(the calibration kernel for HBM counters), then exit")
@@ -162,7 +166,7 @@ def main():
         sys.exit(2)
     # HBM-counter runs first, as child processes, before this process touches the GPU; then the CPU baseline
     # (host only, its one-tank-per-core processes started before the GPU is initialised)
-    side = args.profile_matvec or args.profile_precond or args.batch or args.serve or args.batched_only
+    side = args.profile_matvec or args.profile_precond or args.profile_steps or args.batch or args.serve or args.batched_only
     tr = None if (args.no_traffic or side) else measure_traffic()
     cpu = None if (args.no_cpu or side) else cpu_baseline()
     import ow3d
@@ -180,6 +184,15 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(bench_batched_513(ow3d, problems, args.batched_tanks, args.steps, args.warmup)), flush=True)
+        return
+    if args.profile_steps:
+        T, cfg, dt = make_tank(ow3d, problems, args.precision)
+        run_steps(T, 2, dt)
+        st = []
+        lat = run_steps(T, args.profile_steps, dt, 2 * dt, st)
+        print(json.dumps({"profile_steps": args.profile_steps, "precision": args.precision,
+                          "ms_per_step": 1e3 * float(np.mean(lat)), "status": st}), flush=True)
+        T.close()
         return
     if args.profile_precond:
         T, cfg, dt = make_tank(ow3d, problems, 1)
