@@ -227,7 +227,7 @@ PlaneArgs plane_args(ow3d_tank *t, double *E, double *P) {
 void surface_dev(ow3d_tank *t, double *E, double *P, hipStream_t s) {
     PlaneArgs a = plane_args(t, E, P);
     hipLaunchKernelGGL(k_surface_x, dim3(t->g.nyg), dim3(256), 0, s, a);
-    if (t->g.ny > 1) hipLaunchKernelGGL(k_surface_y, dim3(t->g.nxg), dim3(256), 0, s, a);
+    if (t->g.ny > 1) hipLaunchKernelGGL(k_surface_y, dim3((t->g.nxg + 63) / 64, (t->g.nyg + 3) / 4), dim3(256), 0, s, a);
 }
 
 void wvel_dev(ow3d_tank *t, const double *phi, double *W, hipStream_t s) {

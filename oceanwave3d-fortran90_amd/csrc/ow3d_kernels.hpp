@@ -841,38 +841,47 @@ static __global__ __launch_bounds__(256) void k_surface_x(PlaneArgs g) {
     }
 }
 
-// one block per column i: y ghost values (columns with i in [gx, nxg-gx)), y derivatives, coefficients.
+// y ghost values (columns with i in [gx, nxg-gx)), y derivatives and coefficients: one thread per plane
+// point, 64 consecutive columns x 4 rows per block so that every load and store of a wave is contiguous.
+// A ghost value depends only on interior values of its column, so each thread whose stencil reaches a
+// ghost row evaluates it itself; the threads of the ghost rows also write it back.
 static __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
-    const int i = blockIdx.x, nxg = g.nxg, nyg = g.nyg, rb = 2 * g.b + 1;
-    if (i >= g.gx && i < nxg - g.gx && threadIdx.x < 4) {
-        double *v = (threadIdx.x & 1) ? g.P : g.E;
-        if (threadIdx.x < 2) {
+    const int nxg = g.nxg, nyg = g.nyg, rb = 2 * g.b + 1;
+    const int i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), j = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (i >= nxg || j >= nyg) return;
+    const bool gcol = i >= g.gx && i < nxg - g.gx;
+    auto val = [&](const double *v, int jj) -> double {
+        if (gcol && jj == 0) {
             const double *st = g.sy + (size_t)(1 * 2) * rb;
             double s = 0.0;
             for (int m = 1; m < rb; m++) s += st[m] * v[i + (size_t)nxg * m];
-            v[i] = (0.0 - s) / st[0];
-        } else {
+            return (0.0 - s) / st[0];
+        }
+        if (gcol && jj == nyg - 1) {
             const double *st = g.sy + (size_t)((nyg - 2) * 2) * rb;
             double s = 0.0;
             for (int m = 0; m < rb - 1; m++) s += st[m] * v[i + (size_t)nxg * (nyg - rb + m)];
-            v[i + (size_t)nxg * (nyg - 1)] = (0.0 - s) / st[rb - 1];
+            return (0.0 - s) / st[rb - 1];
         }
+        return v[i + (size_t)nxg * jj];
+    };
+    const int ws = cclamp(j - g.b, 0, nyg - rb);
+    const double *S1 = g.sy + (size_t)(j * 2) * rb, *S2 = S1 + rb;
+    double e1 = 0.0, e2 = 0.0, p1 = 0.0;
+    for (int m = 0; m < rb; m++) {
+        const double ev = val(g.E, ws + m), pv = val(g.P, ws + m);
+        e1 += S1[m] * ev; e2 += S2[m] * ev; p1 += S1[m] * pv;
     }
-    __syncthreads();
-    for (int j = threadIdx.x; j < nyg; j += blockDim.x) {
-        const int ws = cclamp(j - g.b, 0, nyg - rb);
-        const double *S1 = g.sy + (size_t)(j * 2) * rb, *S2 = S1 + rb;
-        double e1 = 0.0, e2 = 0.0, p1 = 0.0;
-        for (int m = 0; m < rb; m++) {
-            const size_t qq = i + (size_t)nxg * (ws + m);
-            e1 += S1[m] * g.E[qq]; e2 += S2[m] * g.E[qq]; p1 += S1[m] * g.P[qq];
-        }
-        const size_t q = i + (size_t)nxg * j;
-        g.Ey[q] = e1; g.Eyy[q] = e2; g.Py[q] = p1;
-        if (g.nonlinear)
-            column_coefs(g.E[q] + g.h[q], g.Ex[q], e1, g.Exx[q] + e2, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
-        else
-            column_coefs(g.h[q], 0.0, 0.0, 0.0, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
+    const size_t q = i + (size_t)nxg * j;
+    const double eq = val(g.E, j);
+    g.Ey[q] = e1; g.Eyy[q] = e2; g.Py[q] = p1;
+    if (g.nonlinear)
+        column_coefs(eq + g.h[q], g.Ex[q], e1, g.Exx[q] + e2, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
+    else
+        column_coefs(g.h[q], 0.0, 0.0, 0.0, g.bath[q], g.bath[g.nxy + q], g.bath[2 * g.nxy + q], g.cf, g.nxy, (int)q);
+    if (gcol && (j == 0 || j == nyg - 1)) {   // every reader above evaluated the ghost value itself
+        const double pq = val(g.P, j);
+        g.E[q] = eq; g.P[q] = pq;
     }
 }
 
