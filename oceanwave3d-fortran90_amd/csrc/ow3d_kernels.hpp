@@ -1411,6 +1411,66 @@ __global__ __launch_bounds__(256) void k_dcgs_b(size_t n, const double *__restri
     block_reduce_store<K + 1, PSTRIDE>(acc, part);
 }
 
+// The two passes on pairs of grid points (16-byte loads and stores), for an even n: the basis vectors then
+// start 16-byte aligned.  Each lane sums its pair's products in order (first point, then second).
+template <int K, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_dcgs_a2(size_t n, double *__restrict__ V, const KrylovState *S,
+                                                  double *__restrict__ part, const int *skip) {
+    static_assert(K >= 2, "the first step has no pending vector");
+    if (solve_done(skip)) return;
+    double c[K - 1], acc[K];
+#pragma unroll
+    for (int i = 0; i < K - 1; i++) c[i] = S->cv[i];
+    const double ib = S->ib;
+#pragma unroll
+    for (int i = 0; i < K; i++) acc[i] = 0.0;
+    double2 *__restrict__ u = reinterpret_cast<double2 *>(V + (size_t)(K - 1) * n);
+    const double2 *__restrict__ z = reinterpret_cast<const double2 *>(V + (size_t)K * n);
+    const size_t n2 = n / 2;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += (size_t)gridDim.x * blockDim.x) {
+        double2 vv[K - 1];
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) vv[i] = reinterpret_cast<const double2 *>(V + (size_t)i * n)[q];
+        double2 x = u[q];
+        const double2 zq = z[q];
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) { x.x -= c[i] * vv[i].x; x.y -= c[i] * vv[i].y; }
+        x.x *= ib; x.y *= ib;
+        u[q] = x;
+#pragma unroll
+        for (int i = 0; i < K - 1; i++) { acc[i] += vv[i].x * zq.x; acc[i] += vv[i].y * zq.y; }
+        acc[K - 1] += x.x * zq.x; acc[K - 1] += x.y * zq.y;
+    }
+    block_reduce_store<K, PSTRIDE>(acc, part);
+}
+template <int K, int PSTRIDE>
+__global__ __launch_bounds__(256) void k_dcgs_b2(size_t n, const double *__restrict__ V, const double *__restrict__ h,
+                                                  const KrylovState *S, double *__restrict__ v, double *__restrict__ part,
+                                                  const int *skip) {
+    if (solve_done(skip)) return;
+    double acc[K + 1], hh[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) { acc[i] = 0.0; hh[i] = h[i]; }
+    acc[K] = 0.0;
+    const double ib = S->ib;
+    double2 *__restrict__ v2 = reinterpret_cast<double2 *>(v);
+    const size_t n2 = n / 2;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += (size_t)gridDim.x * blockDim.x) {
+        double2 vv[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) vv[i] = reinterpret_cast<const double2 *>(V + (size_t)i * n)[q];
+        double2 x = v2[q];
+#pragma unroll
+        for (int i = 0; i < K; i++) { x.x -= hh[i] * vv[i].x; x.y -= hh[i] * vv[i].y; }
+        x.x *= ib; x.y *= ib;
+        v2[q] = x;
+#pragma unroll
+        for (int i = 0; i < K; i++) { acc[i] += vv[i].x * x.x; acc[i] += vv[i].y * x.y; }
+        acc[K] += x.x * x.x; acc[K] += x.y * x.y;
+    }
+    block_reduce_store<K + 1, PSTRIDE>(acc, part);
+}
+
 // Step k: d = the first pass's dots (k), e = the second pass's dots and squared norm (k + 1).  Completes
 // Hessenberg column k - 1, rotates it, updates the residual estimate and the stopping test, and keeps c'
 // and the new inverse norm for the next step.
