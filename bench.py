@@ -356,7 +356,7 @@ def bench_batch(ow3d, problems, ntanks, steps, warmup, seed=0, use_tuning=True):
         for d in draws:
             cfg, bath, E, P, dt, _ = problems.batch_tank_from(d)
             # FP32 multigrid with flexible GMRES: every tank of the mix within 1e-6 of its FP64 run
-            cfg = dict(cfg, mg_precision=prec, gmres_m=23, reltol=1e-8, abstol=0.0, gmres_side=1 if prec else 0)
+            cfg = dict(cfg, mg_precision=prec, reltol=1e-8, abstol=0.0, gmres_side=1 if prec else 0)
             choice = autotune.tuned(cfg, table)
             if choice:
                 cfg["mg_maxgrids"] = choice["mg_maxgrids"]

@@ -15,6 +15,7 @@ input (the solve still runs to its tolerance).  Choices are cached by class in t
 file; tuned(cfg) returns the cached choice or None.
 
     python -m ow3d.autotune --batch 64 --seed 0      # tune the classes of the 64-tank workload
+    python -m ow3d.autotune --batch 64 --workload shoal513 --matvec-only   # re-time the matvec choice only
 """
 import argparse
 import json
@@ -119,6 +120,25 @@ def tune_class(spec, mg_precision=1, warmup=3, steps=3, log=None):
     return choice
 
 
+def retune_matvec(spec, choice, mg_precision=1):
+    """Re-times only the matvec kernels of a class whose schedule is cached (after a kernel change) and
+    returns the choice with the fastest variant."""
+    cfg0, bath, E, P, dt, zones = spec
+    base = dict(cfg0, mg_precision=mg_precision, gmres_m=23, reltol=1e-8, abstol=0.0, mg_maxgrids=choice["mg_maxgrids"])
+    T = ow3d.Tank(base, **bath)
+    T.set_state(E, P)
+    mv = {}
+    for v in ((0, 1, 2, 3) if base["ny"] > 1 else (0,)):
+        T.tune(v, choice["sweep_threads"])
+        T.time_matvec(3)
+        mv[v] = 1e3 * T.time_matvec(30)
+    T.close()
+    out = dict(choice)
+    out["matvec_variant"] = min(mv, key=mv.get)
+    out["matvec_us"] = {str(k): round(v, 2) for k, v in mv.items()}
+    return out
+
+
 def apply(T, choice):
     """Applies a cached choice's launch knobs to a tank built with its mg_maxgrids."""
     if choice:
@@ -133,15 +153,24 @@ def main(argv=None):
     ap.add_argument("--cache", default=CACHE, help="table to read and extend")
     ap.add_argument("--workload", action="append", default=[], help="also tune a problems.workload tank (e.g. shoal513)")
     ap.add_argument("--precisions", type=int, nargs="+", default=[0, 1])
+    ap.add_argument("--matvec-only", action="store_true", help="re-time only the matvec of the cached classes")
     args = ap.parse_args(argv)
     table = load(args.cache)
     specs = [(lambda n=n: problems.workload(n)) for n in args.workload]
     specs += [(lambda d=problems.batch_draw(args.seed * 1000003 + i): problems.batch_tank_from(d)) for i in range(args.batch)]
     cfgs = [problems.workload(n)[0] for n in args.workload]
     cfgs += [problems.batch_cfg(problems.batch_draw(args.seed * 1000003 + i)) for i in range(args.batch)]
+    retimed = set()
     for prec in args.precisions:
         for make_spec, cfg in zip(specs, cfgs):
             key = class_key(dict(cfg, mg_precision=prec))
+            if args.matvec_only:
+                if key in table and key not in retimed:
+                    retimed.add(key)
+                    table[key] = retune_matvec(make_spec(), table[key], mg_precision=prec)
+                    print(f"{key}: matvec {table[key]['matvec_variant']} {table[key]['matvec_us']}", flush=True)
+                    save(table, args.cache)
+                continue
             if key in table and not args.retune:
                 continue
             table[key] = tune_class(make_spec(), mg_precision=prec, log=lambda s: print(s, flush=True))

@@ -192,11 +192,12 @@ def batch_draw(seed, bottom=None, sizes=BATCH_SIZES, nzs=BATCH_NZ, orders=BATCH_
 
 
 def batch_cfg(d):
-    """Tank configuration of a draw: 16 points per peak wavelength (L = (N - 1) / 16 * lambda_p)."""
+    """Tank configuration of a draw: 16 points per peak wavelength (L = (N - 1) / 16 * lambda_p); GMRES(31) for
+    8th-order tanks, whose solves often stop at the 23-step limit unconverged, GMRES(23) otherwise."""
     lam = 2 * np.pi * d["h0"] / d["kph"]
     a = d["order"] // 2
     return dict(nx=d["nx"], ny=d["ny"], nz=d["nz"], lx=(d["nx"] - 1) / 16 * lam, ly=(d["ny"] - 1) / 16 * lam,
-                alpha=a, beta=a, gamma=a)
+                alpha=a, beta=a, gamma=a, gmres_m=31 if d["order"] >= 8 else 23)
 
 
 def batch_tank_from(d):

@@ -48,7 +48,7 @@ class Request:
         cfg = problems.batch_cfg(draw)
         # the FP32 multigrid runs with flexible GMRES, so that every tank of the varied mix stays within 1e-6 of
         # its FP64 run (left preconditioning with an FP32 V cycle does not on some tanks)
-        cfg.update(mg_precision=self.mg_precision, gmres_m=23, reltol=1e-8, abstol=0.0,
+        cfg.update(mg_precision=self.mg_precision, reltol=1e-8, abstol=0.0,
                    gmres_side=1 if self.mg_precision >= 1 else 0)
         self.choice = autotune.tuned(cfg, _TABLE) if tuned else None   # the class's cached schedule
         if self.choice:

@@ -16,6 +16,7 @@ def test_batch_draws_respect_the_workload_ranges():
         cfg = problems.batch_cfg(d)
         lam = 2 * np.pi * d["h0"] / d["kph"]
         assert abs(cfg["lx"] / (d["nx"] - 1) - lam / 16) < 1e-12 * lam   # 16 points per peak wavelength
+        assert cfg["gmres_m"] == (31 if d["order"] == 8 else 23)   # 8th order converges in GMRES(31)
         seen.add((d["nz"], d["order"]))
     assert (6, 8) not in seen and (12, 8) in seen and (6, 4) in seen
     # a draw is a function of its seed
