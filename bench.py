@@ -153,6 +153,7 @@ def main():
     ap.add_argument("--untuned", action="store_true", help="--batch / --serve without the autotuner's cached schedules")
     ap.add_argument("--serve", type=float, default=0.0, metavar="SECONDS",
                     help="request-loop workload instead: Poisson arrivals served for SECONDS (prints its own JSON line)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="with --serve: Poisson arrival rate per second")
     ap.add_argument("--profile-steps", type=int, default=0, metavar="K",
                     help="only run 2 + K RK4 steps of the 513^2 workload with the multigrid precision of "
                          "--precision, then exit (for rocprofv3 kernel traces)")
@@ -180,7 +181,7 @@ def main():
               flush=True)
         return
     if args.serve:
-        print(json.dumps(bench_serve(args.serve, tuned=not args.untuned)), flush=True)
+        print(json.dumps(bench_serve(args.serve, rate=args.serve_rate, tuned=not args.untuned)), flush=True)
         return
     if args.batched_only:
         print(json.dumps(bench_batched_513(ow3d, problems, args.batched_tanks, args.steps, args.warmup)), flush=True)
