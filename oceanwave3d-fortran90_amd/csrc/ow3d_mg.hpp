@@ -342,6 +342,18 @@ void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
     hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks((size_t)G.nxgf * G.nygf * G.nzgf)), dim3(256), 0, s, G, ec, xf);
 }
 
+// Fused sweep on TX x TY tiles with NT threads (the launch grid the kernel's tile arithmetic assumes).
+template <typename T, int NZ, int TX, int TY, int NT>
+void rb_tiles(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s, const double *r0, T *bout, double *uout) {
+    static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
+    const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
+    const int offx = (nxi % TX == 1) ? 1 : 0, offy = (L.dim3 && nyi % TY == 1) ? 1 : 0;
+    const dim3 grid((nxi + offx + TX - 1) / TX, L.dim3 ? (nyi + offy + TY - 1) / TY : 1);
+    if (r0) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 1>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
+    else if (uout) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 2>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
+    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
+}
+
 // Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
 // under it while other blocks write their results).
 template <typename T, int NZ>
@@ -349,25 +361,21 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int thr
                   const double *r0, T *bout, double *uout) {
     // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
-    static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
-    const int offx = (nxi % TX == 1) ? 1 : 0, offy = (L.dim3 && nyi % TY == 1) ? 1 : 0;
-    dim3 grid((nxi + offx + TX - 1) / TX, L.dim3 ? (nyi + offy + TY - 1) / TY : 1);
+    const long tiles = (long)((nxi + TX - 1) / TX) * (L.dim3 ? (nyi + TY - 1) / TY : 1);
+    // FP32 3D levels with many tiles (measured at 513^2 x 10 and 257^2 x 10, kernel medians per sweep):
+    // 16 x 8 tiles of 512 threads on the largest (42.7 us against 49.0 us for 32 x 4 tiles of 1024
+    // threads: with twice the blocks per CU one block's loads overlap another's rows), 32 x 4 tiles of
+    // 512 threads on the next (20.4 against 22.2 us); smaller levels keep one 1024-thread block per tile
+    if constexpr (sizeof(T) == 4 && TX == 32 && rb_lds<T, NZ, 16, 8>() <= 65536) {
+        if (L.dim3 && threads != 256 && tiles >= 1500) { rb_tiles<T, NZ, 16, 8, 512>(L, xin, xout, b, s, r0, bout, uout); return; }
+        if (L.dim3 && threads != 256 && tiles >= 400) { rb_tiles<T, NZ, 32, 4, 512>(L, xin, xout, b, s, r0, bout, uout); return; }
+    }
     // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
-    const bool big = threads == 1024 || (threads == 0 && (long)grid.x * grid.y < 256);
-    if (r0) {   // first sweep from x = 0, b = (T) r0 written to bout
-        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, 1>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
-        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, 1>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
-        return;
-    }
-    if (uout) {   // last sweep: the result goes to uout in FP64
-        if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024, 2>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
-        else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256, 2>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
-        return;
-    }
-    if (big) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 1024>), grid, dim3(1024), 0, s, L, xin, xout, b, r0, bout, uout);
-    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, 256>), grid, dim3(256), 0, s, L, xin, xout, b, r0, bout, uout);
+    const bool big = threads == 1024 || (threads == 0 && tiles < 256);
+    if (big) rb_tiles<T, NZ, TX, TY, 1024>(L, xin, xout, b, s, r0, bout, uout);
+    else rb_tiles<T, NZ, TX, TY, 256>(L, xin, xout, b, s, r0, bout, uout);
 }
 
 // Fused sweep xin -> xout; with r0 (not NULL) the first sweep of an application: x = 0 and b = (T) r0,
