@@ -86,6 +86,13 @@ struct MgLevel {
     DevBuf<__half> minv16;                             // coarsest-level inverse in FP16 (mg_precision 2)
     int minv_pitch = 0;                                // row pitch (elements) of minv / minv16
     DevBuf<double> xf, yf, zf;   // level coordinates (restriction / prolongation weights)
+    // prolongation onto this level from the next coarser one when that one is coarsened in x and y but not
+    // in sigma: per fine column i (row j) the two coarse columns (rows) it interpolates from, -1 outside the
+    // interior, and the weights (left, right, 1 / sum) in T (left = right index for points on a coarse line,
+    // weights (1, 0, 1))
+    DevBuf<int2> pxi, pyi;
+    DevBuf<T> pxw, pyw;
+    bool ptab = false;
     LvArgs<T> args{};
 };
 

@@ -2687,6 +2687,32 @@ __global__ void k_mg_prolong_add_t(RestrictGeo G, const T *__restrict__ ec, T *_
     xf[q] = x0 + e;
 }
 
+// The same prolongation for a coarse level coarsened in x and y but not in sigma (kc = k), from per-column
+// and per-row tables (left and right coarse index, weights left, right and 1 / sum, as prolong_at forms
+// them): e = (wyl (wxl a + wxr b) sx + wyr (wxl c + wxr d) sx) sy over the (up to) four coarse neighbours,
+// which is prolong_at's value with weights (1, 0, 1) on the coarse lines.  No branch on the point's parity.
+template <typename T, int NZF>
+__global__ void k_mg_prolong_tab(RestrictGeo G, const int2 *__restrict__ pxi, const T *__restrict__ pxw, const int2 *__restrict__ pyi,
+                                 const T *__restrict__ pyw, const T *__restrict__ ec, T *__restrict__ xf) {
+    if (!G.spec && solve_done(G.skip)) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= G.nxgf * NZF) return;
+    const int i = t / NZF, k = t - i * NZF, j = (int)blockIdx.y;
+    const int2 yi = pyi[j], xi = pxi[i];
+    if (yi.x < 0 || xi.x < 0 || k < 1 || k > G.nzf - 1) return;
+    const T wxl = pxw[3 * i], wxr = pxw[3 * i + 1], sx = pxw[3 * i + 2];
+    const T wyl = pyw[3 * j], wyr = pyw[3 * j + 1], sy = pyw[3 * j + 2];
+    const size_t rl = (size_t)yi.x * G.nxgc, rr = (size_t)yi.y * G.nxgc;
+    const T a = ec[(rl + xi.x) * NZF + k], b = ec[(rl + xi.y) * NZF + k];
+    const T c = ec[(rr + xi.x) * NZF + k], d = ec[(rr + xi.y) * NZF + k];
+    const T el = (wxl * a + wxr * b) * sx, er = (wxl * c + wxr * d) * sx;
+    const T e = (wyl * el + wyr * er) * sy;
+    const size_t q = (size_t)j * G.nxgf * NZF + t;
+    const T x0 = xf[q];
+    if (G.spec && solve_done(G.skip)) return;
+    xf[q] = x0 + e;
+}
+
 // Coarsest level: x = Minv b with the dense inverse computed at set-up, stored as M (T, or FP16 with
 // sums in T) in rows of `pitch` elements (a multiple of 8; zeros past n); b holds pitch elements (zeros
 // past n).  One 64-lane wave per row: each lane reads 16 bytes of the row and the matching b values per

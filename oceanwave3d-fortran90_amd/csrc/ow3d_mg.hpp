@@ -278,6 +278,41 @@ int mg_build(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg) {
         }
         mg.push_back(std::move(L));
     }
+    // interpolation tables of the prolongations (see MgLevel::ptab): the weights are the ones prolong_at
+    // forms per point, (T) of the double coordinate differences and T(1) / (their sum) in T
+    for (int k = 1; k < t->nlev; k++) {
+        MgLevel<T> &F = *mg[k];
+        const Grid &f = F.g, &c = mg[k - 1]->g;
+        if (!(c.xc && c.yc && !c.zc && f.ny > 1)) continue;
+        auto table = [](int n, int g, int nin, const std::vector<double> &x, std::vector<int2> &idx, std::vector<T> &w) {
+            idx.assign(n, make_int2(-1, -1));
+            w.assign((size_t)3 * n, T(0));
+            for (int i = 0; i < n; i++) {
+                if (i < g || i > nin + g - 1) continue;
+                const int ie = i - g;
+                if (ie % 2 == 0) {
+                    idx[i] = make_int2(g + ie / 2, g + ie / 2);
+                    w[3 * i] = T(1); w[3 * i + 1] = T(0); w[3 * i + 2] = T(1);
+                } else {
+                    const int cl = g + (ie - 1) / 2;
+                    const T dlo = (T)(x[i] - x[i - 1]), dhi = (T)(x[i + 1] - x[i]);
+                    idx[i] = make_int2(cl, cl + 1);
+                    w[3 * i] = dhi; w[3 * i + 1] = dlo; w[3 * i + 2] = T(1) / (dlo + dhi);
+                }
+            }
+        };
+        std::vector<int2> xi, yi;
+        std::vector<T> xw, yw;
+        table(f.nxg, f.gx, f.nx, f.x, xi, xw);
+        table(f.nyg, f.gy, f.ny, f.y, yi, yw);
+        HIPCHK(F.pxi.alloc(xi.size())); HIPCHK(F.pyi.alloc(yi.size()));
+        HIPCHK(F.pxw.alloc(xw.size())); HIPCHK(F.pyw.alloc(yw.size()));
+        HIPCHK(hipMemcpy(F.pxi.p, xi.data(), xi.size() * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.pyi.p, yi.data(), yi.size() * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.pxw.p, xw.data(), xw.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.pyw.p, yw.data(), yw.size() * sizeof(T), hipMemcpyHostToDevice));
+        F.ptab = true;
+    }
     (void)s;
     return 0;
 }
@@ -330,6 +365,18 @@ void restrict_launch(const RestrictGeo &G, const T *rf, T *rc, T *xz, hipStream_
 #undef OW3D_NZ
     }
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks((size_t)G.nxgc * G.nygc * G.nzgc)), dim3(256), 0, s, G, rf, rc, xz);
+}
+// Prolongation onto fine level F from its interpolation tables (F.ptab), else prolong_launch.
+template <typename T>
+void prolong_tab_launch(const RestrictGeo &G, const MgLevel<T> &F, const T *ec, T *xf, hipStream_t s) {
+    switch (G.nzgf) {
+#define OW3D_NZ(N) case N: hipLaunchKernelGGL((k_mg_prolong_tab<T, N>), dim3((G.nxgf * N + 255) / 256, G.nygf), dim3(256), 0, s, G, \
+                                          (const int2 *)F.pxi.p, (const T *)F.pxw.p, (const int2 *)F.pyi.p, (const T *)F.pyw.p, ec, xf); return;
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid_blocks((size_t)G.nxgf * G.nygf * G.nzgf)), dim3(256), 0, s, G, ec, xf);
 }
 template <typename T>
 void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
@@ -459,7 +506,8 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     restrict_launch<T>(G, (const T *)L.r.p, C.b.p, C.x.p, s);
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
-    prolong_launch<T>(G, (const T *)ccur, cur, s);
+    if (L.ptab) prolong_tab_launch<T>(G, L, (const T *)ccur, cur, s);
+    else prolong_launch<T>(G, (const T *)ccur, cur, s);
     if (smooth(t, L, t->cfg.nu2, cur, s, nullptr, uout)) return -1;
     return 0;
 }
