@@ -93,6 +93,9 @@ struct MgLevel {
     DevBuf<int2> pxi, pyi;
     DevBuf<T> pxw, pyw;
     bool ptab = false;
+    // restriction from this level onto the next coarser one in the same case: per coarse column (row) the
+    // weights of the 3-point smoothing on the fine grid, (d_lo, d_lo + d_hi, d_hi, 1 / (2 (d_lo + d_hi))) in T
+    DevBuf<T> rxw, ryw;
     LvArgs<T> args{};
 };
 

@@ -2687,6 +2687,40 @@ __global__ void k_mg_prolong_add_t(RestrictGeo G, const T *__restrict__ ec, T *_
     xf[q] = x0 + e;
 }
 
+// Restriction onto a coarse level coarsened in x and y but not in sigma (kc = kf), with the 3-point weights
+// of each coarse column and row from tables (restrict_at's values: the fine residual smoothed along x on
+// the coarse point's row and its two neighbour rows, injected on wall ghost rows, then along y; injected on
+// the top level; zero on coarse ghost points).  Also clears the coarse iterate.
+template <typename T, int NZC>
+__global__ void k_mg_restrict_tab(RestrictGeo G, const T *__restrict__ rxw, const T *__restrict__ ryw, const T *__restrict__ rf,
+                                  T *__restrict__ rc, T *__restrict__ xz) {
+    if (!G.spec && solve_done(G.skip)) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= G.nxgc * NZC) return;
+    const int ic = t / NZC, k = t - ic * NZC, jc = (int)blockIdx.y;
+    const int icn = ic - G.gx, jcn = jc - G.gy;
+    T v = T(0);
+    if (k >= 1 && icn >= 0 && icn < G.nxc && jcn >= 0 && jcn < G.nyc) {
+        const int i = G.gx + 2 * icn, j = G.gy + 2 * jcn;
+        auto R = [&](int ii, int jj) -> T { return rf[((size_t)jj * G.nxgf + ii) * NZC + k]; };
+        if (k <= G.nzf - 1) {
+            const T x0 = rxw[4 * ic], x1 = rxw[4 * ic + 1], x2 = rxw[4 * ic + 2], ax = rxw[4 * ic + 3];
+            auto Xv = [&](int jj) -> T {
+                if (jj < G.gy || jj > G.nyf + G.gy - 1) return R(i, jj);
+                return ax * ((x0 * R(i - 1, jj) + x1 * R(i, jj)) + x2 * R(i + 1, jj));
+            };
+            const T y0 = ryw[4 * jc], y1 = ryw[4 * jc + 1], y2 = ryw[4 * jc + 2], ay = ryw[4 * jc + 3];
+            v = ay * ((y0 * Xv(j - 1) + y1 * Xv(j)) + y2 * Xv(j + 1));
+        } else {
+            v = R(i, j);
+        }
+    }
+    if (G.spec && solve_done(G.skip)) return;
+    const size_t q = (size_t)jc * G.nxgc * NZC + t;
+    xz[q] = T(0);
+    rc[q] = v;
+}
+
 // The same prolongation for a coarse level coarsened in x and y but not in sigma (kc = k), from per-column
 // and per-row tables (left and right coarse index, weights left, right and 1 / sum, as prolong_at forms
 // them): e = (wyl (wxl a + wxr b) sx + wyr (wxl c + wxr d) sx) sy over the (up to) four coarse neighbours,
