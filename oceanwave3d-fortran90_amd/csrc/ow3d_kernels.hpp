@@ -471,20 +471,21 @@ __global__ __launch_bounds__(16 * TY) void k_matvec3c(MvArgs g, const double *__
     }
 }
 
-// The column matvec with the work of each column split between two waves: a block of 16 x TY columns
-// has 32 TY threads; the first 16 TY take the x-side of column t (d/dx, d2/dx2, the first sigma
+// The column matvec with the work of each column split between two waves: a block of TX x TY columns
+// has 2 TX TY threads; the first TX TY take the x-side of column t (d/dx, d2/dx2, the first sigma
 // derivative of phi and the mixed x-sigma term, the bottom row), the others the y-side of column
-// t - 16 TY (d/dy, d2/dy2, the second sigma derivative and the mixed y-sigma term): four
+// t - TX TY (d/dy, d2/dy2, the second sigma derivative and the mixed y-sigma term): four
 // 2A+1-point stencils per level on each side.  The y-side leaves its part of every row in LDS; the x-side
 // adds it.  Each thread holds half the live values of the one-thread-per-column kernel and a block has
 // twice the waves.  Same terms as k_matvec3c; the x and y parts are summed in another order (results agree
-// to rounding).  The live values of a column take about 210 VGPRs (two waves per SIMD): with 16 x 8
-// tiles (4 waves, 44 KiB of LDS at 10 levels) two blocks share a CU, so one block's window loads overlap
-// the other's arithmetic (29.7 us against 36.3 us for one 16 x 16 block per CU on a 515 x 515 x 10 grid).
-template <int NZ, int A, int TY = 8>
-__global__ __launch_bounds__(32 * TY) void k_matvec3s(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
+// to rounding).  The live values of a column take about 210 VGPRs (two waves per SIMD), so a block of
+// 4 waves lets two blocks share a CU and one block's window loads overlap the other's arithmetic.  On a
+// 515 x 515 x 10 grid (6th order): 16 x 8 tiles 29.6 us, 32 x 4 32.5 us, 16 x 4 33.1 us, 16 x 16 (one
+// block per CU) 36.3 us.
+template <int NZ, int A, int TX = 16, int TY = 8>
+__global__ __launch_bounds__(2 * TX * TY) void k_matvec3s(MvArgs g, const double *__restrict__ phi, double *__restrict__ out) {
     if (solve_done(g.skip)) return;
-    constexpr int TX = 16, NC = TX * TY, NT = 2 * NC, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
+    constexpr int NC = TX * TY, NT = 2 * NC, RA = 2 * A + 1, R = A + 1, WX = TX + 2 * R, WY = TY + 2 * R;
     constexpr int PS = (WX * WY) | 1;
     __shared__ double sP[NZ * PS];
     __shared__ double sSX[(TX + 2) * 2 * RA];

@@ -35,17 +35,21 @@ int launch_mv3c_ty(const ow3d_tank *t, const MvArgs &a, const double *phi, doubl
     hipLaunchKernelGGL((k_matvec3c<NZ, A, TY>), dim3(nb), dim3(16 * TY), 0, s, a, phi, y);
     return 0;
 }
-template <int NZ, int A, int TY>
+template <int NZ, int A, int TX, int TY>
 constexpr size_t mv3s_lds() {
     constexpr int R = A + 1, RA = 2 * A + 1;
-    return ((size_t)NZ * (((16 + 2 * R) * (TY + 2 * R)) | 1) + (18 + TY + 2) * 2 * RA + NZ * 16 * TY) * 8;
+    return ((size_t)NZ * (((TX + 2 * R) * (TY + 2 * R)) | 1) + (TX + 2 + TY + 2) * 2 * RA + NZ * TX * TY) * 8;
 }
-template <int NZ, int A, int TY>
-int launch_mv3s(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
-    static_assert(mv3s_lds<NZ, A, TY>() <= 80 * 1024, "split matvec tile exceeds half the LDS of a CU");
-    const int nb = ((t->g.nxg + 15) / 16) * ((t->g.nyg + TY - 1) / TY);
-    hipLaunchKernelGGL((k_matvec3s<NZ, A, TY>), dim3(nb), dim3(32 * TY), 0, s, a, phi, y);
+template <int NZ, int A, int TX, int TY>
+int launch_mv3s_t(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
+    static_assert(mv3s_lds<NZ, A, TX, TY>() <= 80 * 1024, "split matvec tile exceeds half the LDS of a CU");
+    const int nb = ((t->g.nxg + TX - 1) / TX) * ((t->g.nyg + TY - 1) / TY);
+    hipLaunchKernelGGL((k_matvec3s<NZ, A, TX, TY>), dim3(nb), dim3(2 * TX * TY), 0, s, a, phi, y);
     return 0;
+}
+template <int NZ, int A>
+int launch_mv3s(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y, hipStream_t s) {
+    return launch_mv3s_t<NZ, A, 16, 8>(t, a, phi, y, s);
 }
 // matvec variant (ow3d_tune): 0 column kernel with the widest tile that fits 64 KiB of LDS, 1 column kernel
 // with 16 x 8 tiles, 2 point kernel (k_matvec3p), 3 column kernel with each column split over two waves
@@ -55,7 +59,7 @@ int launch_mv3(const ow3d_tank *t, const MvArgs &a, const double *phi, double *y
     constexpr int TY = mv3c_lds<NZ, A, 16>() <= 65536 ? 16 : 8;
     if (t->mv_variant == 0) return launch_mv3c_ty<NZ, A, TY>(t, a, phi, y, s);
     if (t->mv_variant == 1) return launch_mv3c_ty<NZ, A, 8>(t, a, phi, y, s);
-    if (t->mv_variant == 3) return launch_mv3s<NZ, A, 8>(t, a, phi, y, s);
+    if (t->mv_variant == 3) return launch_mv3s<NZ, A>(t, a, phi, y, s);
     return launch_mv3p<NZ, A>(t, a, phi, y, s);
 }
 
