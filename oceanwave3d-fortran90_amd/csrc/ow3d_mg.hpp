@@ -560,10 +560,7 @@ int precond_body(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, con
     // with fused sweeps and pre-smoothing above the coarsest level, the first sweep sets b = (T) r from
     // x = 0 itself; otherwise one pass does it first
     const bool fused = t->fused_rb && top >= 1 && top != t->bot_top;
-    // (the fused first sweep, which stages the FP64 residual itself, is the faster form for an FP64
-    // hierarchy; for FP32 the separate conversion pass and a plain sweep are: 27.3 against 28.0 ms per RK4
-    // step of the 513^2 x 9 workload)
-    const bool fuse_in = fused && t->cfg.nu1 >= 1 && sizeof(T) == 8, fuse_out = fused && u && t->cfg.nu2 >= 1;
+    const bool fuse_in = fused && t->cfg.nu1 >= 1, fuse_out = fused && u && t->cfg.nu2 >= 1;
     if (!fuse_in)
         hipLaunchKernelGGL((k_precond_in<T>), dim3(grid_blocks(n)), dim3(256), 0, s, n, r, F.b.p, F.x.p, t->done());
     T *cur = F.x.p;
