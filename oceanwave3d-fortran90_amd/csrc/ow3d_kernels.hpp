@@ -2722,6 +2722,104 @@ __global__ void k_mg_restrict_tab(RestrictGeo G, const T *__restrict__ rxw, cons
     rc[q] = v;
 }
 
+// LDS elements of k_mg_residual_restrict's window (x, then the residual tile over it), coefficients and
+// restriction weights.
+template <int NZ, int CX, int CY>
+struct ResRestrictLds {
+    static constexpr int FW = 2 * CX + 1, FH = 2 * CY + 1, WX = FW + 4, WY = FH + 4, NZP = NZ | 1;
+    static constexpr int X = WX * WY * NZP, C = CoefLds<WX, WY, NZ>::TOTAL, W = 4 * (CX + CY);
+    static constexpr int TOTAL = X + C + W;
+};
+// Residual of a level and its restriction onto a coarse level coarsened in x and y but not in sigma, in one
+// launch (k_mg_residual_t followed by k_mg_restrict_tab, same arithmetic).  A block owns CX x CY coarse
+// columns (ghost columns included); it computes the fine residual (residual_column) on the (2 CX + 1) x
+// (2 CY + 1) fine columns their restriction stencils reach, one thread per column (the fine column between
+// two tiles is computed by both), keeps it in LDS over the staged x window and restricts from there, so the
+// fine residual never goes to HBM.  Also clears the coarse iterate.  Requires wall ghost layers in x and y
+// (gx = gy = 1), as every level with restriction tables has.
+template <typename T, int NZ, int CX, int CY, int NT>
+__global__ __launch_bounds__(NT) void k_mg_residual_restrict(LvArgs<T> L, RestrictGeo G, const T *__restrict__ rxw, const T *__restrict__ ryw,
+                                                             const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ rc,
+                                                             T *__restrict__ xz) {
+    if (!L.spec && solve_done(L.skip)) return;
+    using S = ResRestrictLds<NZ, CX, CY>;
+    constexpr int FW = S::FW, FH = S::FH, WX = S::WX, WY = S::WY, NZP = S::NZP, R = 2;
+    static_assert(FW * FH <= NT, "one thread per fine column of the tile");
+    static_assert(4 * (CX + CY) <= NT, "one weight per thread");
+    __shared__ T sX[S::X];
+    __shared__ T sC[S::C];
+    __shared__ T sW[S::W];
+    const int nxg = L.nxg, nyg = L.nyg, t = (int)threadIdx.x;
+    const int ic0 = (int)blockIdx.x * CX, jc0 = (int)blockIdx.y * CY;
+    // fine column of coarse column ic is 2 ic - 1; the tile's stencils reach fine columns fi0 .. fi0 + FW - 1
+    const int fi0 = 2 * ic0 - 2, fj0 = 2 * jc0 - 2;
+    const int lox = cmax(cmin(fi0 - R, nxg - 4), 0), loy = cmax(cmin(fj0 - R, nyg - 4), 0);
+    const int wcols = cmin(WX, nxg - lox), wrows = cmin(WY, nyg - loy);
+    const int tx = t % FW, ty = t / FW;
+    const int i = fi0 + tx, j = fj0 + ty;
+    const bool active = t < FW * FH && i >= 0 && i < nxg && j >= 0 && j < nyg;
+    // every HBM load of the block (coefficients, x window, the thread's column of b, the tile's restriction
+    // weights) is in flight before the first wait
+    T bc[NZ];
+    const T *bp = b + (active ? (size_t)i + (size_t)nxg * j : (size_t)0) * NZ;
+#pragma unroll
+    for (int k = 0; k < NZ; k++) bc[k] = active ? bp[k] : T(0);
+    T w = T(0);
+    if (t < 4 * CX) {
+        if (ic0 + t / 4 < G.nxgc) w = rxw[4 * ic0 + t];
+    } else if (t < 4 * (CX + CY)) {
+        if (jc0 + (t - 4 * CX) / 4 < G.nygc) w = ryw[4 * jc0 + t - 4 * CX];
+    }
+    CoefRegs<T, NZ, WX, WY, NT> cr;
+    WindowRegs<T, NZ, WY, WX, NT> xr;
+    cr.load(L, lox, loy, wcols, wrows);
+    xr.load(x + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
+    if (L.spec && solve_done(L.skip)) return;
+    const CoefS<T> C = cr.store(sC);
+    xr.template store<NZP, WX>(sX, wrows, wcols);
+    if (t < 4 * (CX + CY)) sW[t] = w;
+    __syncthreads();
+    auto X = [&](int k, int ii, int jj) -> T { return sX[((jj - loy) * WX + (ii - lox)) * NZP + k]; };
+    T o[NZ];
+    if (active) residual_column<T, NZ>(L, C, i, j, X, bc, o);
+    __syncthreads();
+    if (active) {
+        T *dst = sX + (ty * FW + tx) * NZP;
+#pragma unroll
+        for (int k = 0; k < NZ; k++) dst[k] = o[k];
+    }
+    __syncthreads();
+    // restriction (k_mg_restrict_tab's arithmetic) of the tile's coarse points, CX * NZ contiguous values
+    // per coarse row
+    for (int e = t; e < CY * CX * NZ; e += NT) {
+        const int rr = e / (CX * NZ), c = e - rr * (CX * NZ), cc = c / NZ, k = c - cc * NZ;
+        const int ic = ic0 + cc, jc = jc0 + rr;
+        if (ic >= G.nxgc || jc >= G.nygc) continue;
+        const int icn = ic - 1, jcn = jc - 1;
+        T v = T(0);
+        if (k >= 1 && icn >= 0 && icn < G.nxc && jcn >= 0 && jcn < G.nyc) {
+            const int ti = 2 * cc + 1, tj = 2 * rr + 1;   // tile position of fine column (2 ic - 1, 2 jc - 1)
+            auto Rt = [&](int a, int bb) -> T { return sX[(bb * FW + a) * NZP + k]; };
+            if (k <= G.nzf - 1) {
+                const T x0 = sW[4 * cc], x1 = sW[4 * cc + 1], x2 = sW[4 * cc + 2], ax = sW[4 * cc + 3];
+                const int jf = 2 * jc - 1;
+                auto Xv = [&](int dj) -> T {
+                    const int jj = jf + dj;
+                    if (jj < 1 || jj > G.nyf) return Rt(ti, tj + dj);
+                    return ax * ((x0 * Rt(ti - 1, tj + dj) + x1 * Rt(ti, tj + dj)) + x2 * Rt(ti + 1, tj + dj));
+                };
+                const T *Y = sW + 4 * CX + 4 * rr;
+                v = Y[3] * ((Y[0] * Xv(-1) + Y[1] * Xv(0)) + Y[2] * Xv(1));
+            } else {
+                v = Rt(ti, tj);
+            }
+        }
+        const size_t q = ((size_t)jc * G.nxgc + ic) * NZ + k;
+        xz[q] = T(0);
+        rc[q] = v;
+    }
+}
+
 // The same prolongation for a coarse level coarsened in x and y but not in sigma (kc = k), from per-column
 // and per-row tables (left and right coarse index, weights left, right and 1 / sum, as prolong_at forms
 // them): e = (wyl (wxl a + wxr b) sx + wyr (wxl c + wxr d) sx) sy over the (up to) four coarse neighbours,

@@ -399,6 +399,30 @@ void restrict_tab_launch(const RestrictGeo &G, const MgLevel<T> &F, const T *rf,
     }
     hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid_blocks((size_t)G.nxgc * G.nygc * G.nzgc)), dim3(256), 0, s, G, rf, rc, xz);
 }
+// Residual of level F and its restriction in one launch (k_mg_residual_restrict, F.ptab levels), for the
+// column depths whose tile fits 64 KiB of LDS; false otherwise (the caller then launches the two).  Coarse
+// tiles of 8 x 7 columns: 17 x 15 fine columns, one per thread of 256.
+template <typename T, int NZ>
+bool resrestrict_nz(const MgLevel<T> &F, const RestrictGeo &G, const T *x, T *rc, T *xz, hipStream_t s) {
+    constexpr int CX = 8, CY = 7;
+    if constexpr (ResRestrictLds<NZ, CX, CY>::TOTAL * sizeof(T) <= 65536) {
+        hipLaunchKernelGGL((k_mg_residual_restrict<T, NZ, CX, CY, 256>), dim3((G.nxgc + CX - 1) / CX, (G.nygc + CY - 1) / CY), dim3(256), 0, s,
+                           F.args, G, (const T *)F.rxw.p, (const T *)F.ryw.p, x, (const T *)F.b.p, rc, xz);
+        return true;
+    }
+    return false;
+}
+template <typename T>
+bool resrestrict_launch(const MgLevel<T> &F, const RestrictGeo &G, const T *x, T *rc, T *xz, hipStream_t s) {
+    if (!F.ptab || !F.args.dim3 || G.gx != 1 || G.gy != 1 || G.nzgc != F.g.nzg) return false;
+    switch (F.g.nzg) {
+#define OW3D_NZ(N) case N: return resrestrict_nz<T, N>(F, G, x, rc, xz, s);
+        OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
+        OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
+#undef OW3D_NZ
+    }
+    return false;
+}
 // Prolongation onto fine level F from its interpolation tables (F.ptab), else prolong_launch.
 template <typename T>
 void prolong_tab_launch(const RestrictGeo &G, const MgLevel<T> &F, const T *ec, T *xf, hipStream_t s) {
@@ -533,11 +557,13 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     }
     MgLevel<T> &C = *mg[k - 1];
     if (smooth(t, L, t->cfg.nu1, cur, s, r0)) return -1;
-    if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
     RestrictGeo G = geo(L.g, C.g, L.xf.p, L.yf.p, L.zf.p, t->done());
     G.spec = L.args.spec;
-    if (L.ptab) restrict_tab_launch<T>(G, L, (const T *)L.r.p, C.b.p, C.x.p, s);
-    else restrict_launch<T>(G, (const T *)L.r.p, C.b.p, C.x.p, s);
+    if (!resrestrict_launch<T>(L, G, cur, C.b.p, C.x.p, s)) {
+        if (mg_launch<T>(L.args, 0, 0, cur, L.b.p, L.r.p, s)) return -1;
+        if (L.ptab) restrict_tab_launch<T>(G, L, (const T *)L.r.p, C.b.p, C.x.p, s);
+        else restrict_launch<T>(G, (const T *)L.r.p, C.b.p, C.x.p, s);
+    }
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
     if (L.ptab) prolong_tab_launch<T>(G, L, (const T *)ccur, cur, s);
