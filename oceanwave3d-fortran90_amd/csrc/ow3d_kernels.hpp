@@ -2281,18 +2281,41 @@ __global__ __launch_bounds__(128) void k_mg_zebra_t(LvArgs<T> L, int colour, T *
 // the level; this replaces the separate pass b = (T) r, x = 0 before the V cycle.
 // MODE 2: the last sweep of an application: the owned points of the result go to uout as FP64 (the
 // preconditioner's output) instead of to xout; this replaces the separate output conversion.
+// Prolongation fused into the first post-smoothing sweep of a level with interpolation tables
+// (k_mg_prolong_tab's tables and arithmetic; the coarse level is coarsened in x and y, not in sigma).
+template <typename T>
+struct ProArgs {
+    const T *ec;              // coarse correction
+    const int2 *pxi, *pyi;    // per fine column / row: left and right coarse index (-1: none)
+    const T *pxw, *pyw;       // per fine column / row: left weight, right weight, 1 / sum
+    int nxgc;                 // coarse row length
+};
+// Coarse window of a fine window WX x WY: fine interior column i >= 1 interpolates from coarse columns
+// 1 + (i - 1) / 2 and 1 + i / 2.
+template <int WX, int WY>
+struct ProWindow {
+    static constexpr int CWX = WX / 2 + 2, CWY = WY / 2 + 2;
+};
+
 template <typename T, int NZ, int TX, int TY, int NT, int MODE = 0>
 __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__restrict__ xin, T *__restrict__ xout, const T *__restrict__ b,
-                                                    const double *__restrict__ r0, T *__restrict__ bout, double *__restrict__ uout) {
-    constexpr bool ZX = MODE == 1, ZO = MODE == 2;
+                                                    const double *__restrict__ r0, T *__restrict__ bout, double *__restrict__ uout,
+                                                    ProArgs<T> P) {
+    // MODE 1: x = 0, b from r0; MODE 2: result to uout in FP64; MODE 3: the coarse correction P is added to
+    // x first (prolongation); MODE 4: MODE 3 with the result to uout
+    constexpr bool ZX = MODE == 1, ZO = MODE == 2 || MODE == 4, PRO = MODE == 3 || MODE == 4;
     if (!L.spec && solve_done(L.skip)) return;
     constexpr int H = 2, WX = TX + 2 * H, WY = TY + 2 * H, NZP = NZ | 1;
     constexpr int PR = (TX + 2) / 2, NS = PR * (TY + 2);   // red slots: ring-extended tile
+    constexpr int CWX = ProWindow<WX, WY>::CWX, CWY = ProWindow<WX, WY>::CWY;
     __shared__ T sX[WX * WY * NZP];
     __shared__ T sC[CoefLds<WX, WY, NZ>::TOTAL];
     __shared__ T sR[4 * NS * NZ];
     __shared__ T sE[NS];
     __shared__ T sB[(TX + 2) * (TY + 2) * NZ];   // right-hand sides of the ring-extended tile
+    __shared__ T sP[PRO ? CWX * CWY * NZP : 1];              // coarse correction window
+    __shared__ int2 sPi[PRO ? WX + WY : 1];                  // interpolation tables of the window's columns, rows
+    __shared__ T sPw[PRO ? 3 * (WX + WY) : 1];
     const int nxg = L.nxg, nyg = L.nyg, d3 = L.dim3;
     // tiles of TX x TY interior columns; when a direction's count leaves a one-column remainder the first
     // tile is one column short instead, so every tile next to a wall has the two columns its ghost
@@ -2311,11 +2334,28 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     CoefRegs<T, NZ, WX, WY, NT> cr;
     WindowRegs<T, NZ, WY, ZX ? 1 : WX, NT> xr;   // unused when ZX
     WindowRegs<typename std::conditional<ZX, double, T>::type, NZ, TY + 2, TX + 2, NT> br;
+    WindowRegs<T, NZ, PRO ? CWY : 1, PRO ? CWX : 1, NT> pr;   // unused unless PRO
     cr.load(L, lox, loy, wcols, wrows);
     if constexpr (ZX) br.load(r0 + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
     else {
         xr.load(xin + ((size_t)loy * nxg + lox) * NZ, (size_t)nxg * NZ, wrows, wcols);
         br.load(b + ((size_t)bloy * nxg + blox) * NZ, (size_t)nxg * NZ, brows, bcols);
+    }
+    // coarse columns [cxo, cxo + ccols) x rows [cyo, cyo + crows) reach the window's interior columns
+    const int cxo = ((cmax(lox, 1) - 1) >> 1) + 1, cyo = ((cmax(loy, 1) - 1) >> 1) + 1;
+    const int ccols = (cmin(lox + wcols - 1, nxg - 2) >> 1) + 2 - cxo, crows = (cmin(loy + wrows - 1, nyg - 2) >> 1) + 2 - cyo;
+    int2 pti = make_int2(-1, -1);
+    T ptw0 = T(0), ptw1 = T(0), ptw2 = T(0);
+    if constexpr (PRO) {
+        const int t = (int)threadIdx.x;
+        if (t < wcols) {
+            pti = P.pxi[lox + t];
+            ptw0 = P.pxw[3 * (lox + t)]; ptw1 = P.pxw[3 * (lox + t) + 1]; ptw2 = P.pxw[3 * (lox + t) + 2];
+        } else if (t >= WX && t < WX + wrows) {
+            pti = P.pyi[loy + t - WX];
+            ptw0 = P.pyw[3 * (loy + t - WX)]; ptw1 = P.pyw[3 * (loy + t - WX) + 1]; ptw2 = P.pyw[3 * (loy + t - WX) + 2];
+        }
+        pr.load(P.ec + ((size_t)cyo * P.nxgc + cxo) * NZ, (size_t)P.nxgc * NZ, crows, ccols);
     }
     if (L.spec && solve_done(L.skip)) return;
     const CoefS<T> C = cr.store(sC);
@@ -2325,7 +2365,37 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
         xr.template store<NZP, WX>(sX, wrows, wcols);
     }
     br.template store<NZ, TX + 2>(sB, brows, bcols);
+    if constexpr (PRO) {
+        const int t = (int)threadIdx.x;
+        if (t < WX + WY) {
+            sPi[t] = pti;
+            sPw[3 * t] = ptw0; sPw[3 * t + 1] = ptw1; sPw[3 * t + 2] = ptw2;
+        }
+        pr.template store<NZP, CWX>(sP, crows, ccols);
+    }
     __syncthreads();
+    if constexpr (PRO) {
+        // x += interpolated coarse correction on the window's interior columns, sigma levels 1 .. NZ - 2
+        // (k_mg_prolong_tab's arithmetic), before the ghost pass reads them
+        for (int q = threadIdx.x; q < WX * WY; q += NT) {
+            const int r = q / WX, c = q - r * WX;
+            if (r >= wrows || c >= wcols) continue;
+            const int2 xi = sPi[c], yi = sPi[WX + r];
+            if (xi.x < 0 || yi.x < 0) continue;
+            const T wxl = sPw[3 * c], wxr = sPw[3 * c + 1], sxw = sPw[3 * c + 2];
+            const T wyl = sPw[3 * (WX + r)], wyr = sPw[3 * (WX + r) + 1], syw = sPw[3 * (WX + r) + 2];
+            const T *ea = sP + ((yi.x - cyo) * CWX + (xi.x - cxo)) * NZP, *eb = sP + ((yi.x - cyo) * CWX + (xi.y - cxo)) * NZP;
+            const T *ec = sP + ((yi.y - cyo) * CWX + (xi.x - cxo)) * NZP, *ed = sP + ((yi.y - cyo) * CWX + (xi.y - cxo)) * NZP;
+            T *xc = sX + (r * WX + c) * NZP;
+#pragma unroll
+            for (int k = 1; k < NZ - 1; k++) {
+                const T el = (wxl * ea[k] + wxr * eb[k]) * sxw, er = (wxl * ec[k] + wxr * ed[k]) * sxw;
+                const T e = (wyl * el + wyr * er) * syw;
+                xc[k] = xc[k] + e;
+            }
+        }
+        __syncthreads();
+    }
     // owned points [gi0, gi1) x [gj0, gj1): the tile's columns and the ghost points beside them
     const int gi0 = (i0 == L.gx) ? 0 : i0, gi1 = (ie == nxg - L.gx) ? nxg : ie;
     const int gj0 = d3 ? ((j0 == L.gy) ? 0 : j0) : 0, gj1 = d3 ? ((je == nyg - L.gy) ? nyg : je) : 1;

@@ -20,6 +20,12 @@ constexpr size_t rb_lds() {
     return ((size_t)(TX + 4) * (TY + 4) * (NZ | 1) + CoefLds<TX + 4, TY + 4, NZ>::TOTAL + 4 * NS * NZ + NS +
             (size_t)(TX + 2) * (TY + 2) * NZ) * sizeof(T);
 }
+// The same with the prolongation's coarse window and tables (MODE 3 and 4).
+template <typename T, int NZ, int TX, int TY>
+constexpr size_t rb_lds_pro() {
+    using W = ProWindow<TX + 4, TY + 4>;
+    return rb_lds<T, NZ, TX, TY>() + (size_t)W::CWX * W::CWY * (NZ | 1) * sizeof(T) + (size_t)(TX + TY + 8) * (sizeof(int2) + 3 * sizeof(T));
+}
 
 template <typename T, int NZ>
 void mg_launch_nz(const LvArgs<T> &L, int what, int colour, T *x, const T *b, T *r, hipStream_t s) {
@@ -447,22 +453,35 @@ void prolong_launch(const RestrictGeo &G, const T *ec, T *xf, hipStream_t s) {
 }
 
 // Fused sweep on TX x TY tiles with NT threads (the launch grid the kernel's tile arithmetic assumes).
+// With pro (not NULL) the sweep first adds the prolongated coarse correction; false (nothing launched)
+// when that tile does not fit 64 KiB of LDS.
 template <typename T, int NZ, int TX, int TY, int NT>
-void rb_tiles(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s, const double *r0, T *bout, double *uout) {
+bool rb_tiles(const LvArgs<T> &L, const T *xin, T *xout, const T *b, hipStream_t s, const double *r0, T *bout, double *uout,
+              const ProArgs<T> *pro) {
     static_assert(rb_lds<T, NZ, TX, TY>() <= 65536, "zebra sweep tile exceeds 64 KiB of LDS");
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
     const int offx = (nxi % TX == 1) ? 1 : 0, offy = (L.dim3 && nyi % TY == 1) ? 1 : 0;
     const dim3 grid((nxi + offx + TX - 1) / TX, L.dim3 ? (nyi + offy + TY - 1) / TY : 1);
-    if (r0) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 1>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
-    else if (uout) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 2>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
-    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout);
+    const ProArgs<T> P = pro ? *pro : ProArgs<T>{};
+    if (pro) {
+        if constexpr (rb_lds_pro<T, NZ, TX, TY>() <= 65536) {
+            if (uout) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 4>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout, P);
+            else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 3>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout, P);
+            return true;
+        }
+        return false;
+    }
+    if (r0) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 1>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout, P);
+    else if (uout) hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT, 2>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout, P);
+    else hipLaunchKernelGGL((k_mg_zebra_rb<T, NZ, TX, TY, NT>), grid, dim3(NT), 0, s, L, xin, xout, b, r0, bout, uout, P);
+    return true;
 }
 
 // Fused red-black sweep from xin into xout (out of place: a block's staged neighbours must not change
 // under it while other blocks write their results).
 template <typename T, int NZ>
-void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
-                  const double *r0, T *bout, double *uout) {
+bool rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
+                  const double *r0, T *bout, double *uout, const ProArgs<T> *pro) {
     // the widest tile (32, 16 or 8 columns, 4 rows) whose staged data fits in 56 KiB of LDS
     constexpr int TY = 4, TX = rb_lds<T, NZ, 32, TY>() <= 57344 ? 32 : (rb_lds<T, NZ, 16, TY>() <= 57344 ? 16 : 8);
     const int nxi = L.nxg - 2 * L.gx, nyi = L.dim3 ? L.nyg - 2 * L.gy : 1;
@@ -472,24 +491,26 @@ void rb_launch_nz(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int thr
     // threads: with twice the blocks per CU one block's loads overlap another's rows), 32 x 4 tiles of
     // 512 threads on the next (20.4 against 22.2 us); smaller levels keep one 1024-thread block per tile
     if constexpr (sizeof(T) == 4 && TX == 32 && rb_lds<T, NZ, 16, 8>() <= 65536) {
-        if (L.dim3 && threads != 256 && tiles >= 1500) { rb_tiles<T, NZ, 16, 8, 512>(L, xin, xout, b, s, r0, bout, uout); return; }
-        if (L.dim3 && threads != 256 && tiles >= 400) { rb_tiles<T, NZ, 32, 4, 512>(L, xin, xout, b, s, r0, bout, uout); return; }
+        if (L.dim3 && threads != 256 && tiles >= 1500) return rb_tiles<T, NZ, 16, 8, 512>(L, xin, xout, b, s, r0, bout, uout, pro);
+        if (L.dim3 && threads != 256 && tiles >= 400) return rb_tiles<T, NZ, 32, 4, 512>(L, xin, xout, b, s, r0, bout, uout, pro);
     }
     // levels with fewer tiles than CUs are latency-bound: give each tile 1024 threads (about one row of
     // the line systems per thread); larger levels use 256-thread blocks for occupancy
     const bool big = threads == 1024 || (threads == 0 && tiles < 256);
-    if (big) rb_tiles<T, NZ, TX, TY, 1024>(L, xin, xout, b, s, r0, bout, uout);
-    else rb_tiles<T, NZ, TX, TY, 256>(L, xin, xout, b, s, r0, bout, uout);
+    if (big) return rb_tiles<T, NZ, TX, TY, 1024>(L, xin, xout, b, s, r0, bout, uout, pro);
+    return rb_tiles<T, NZ, TX, TY, 256>(L, xin, xout, b, s, r0, bout, uout, pro);
 }
 
 // Fused sweep xin -> xout; with r0 (not NULL) the first sweep of an application: x = 0 and b = (T) r0,
 // written to bout (the level's b); with uout (not NULL) the last: the result goes to uout in FP64 and
-// xout is not written.
+// xout is not written; with pro (not NULL) the prolongated coarse correction is added to xin's values
+// first.  Returns 0 when launched, 1 when the prolongating form does not fit (nothing launched), -1 for
+// a column depth outside 3..14.
 template <typename T>
 int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads, hipStream_t s,
-              const double *r0 = nullptr, T *bout = nullptr, double *uout = nullptr) {
+              const double *r0 = nullptr, T *bout = nullptr, double *uout = nullptr, const ProArgs<T> *pro = nullptr) {
     switch (L.nzg) {
-#define OW3D_NZ(N) case N: rb_launch_nz<T, N>(L, xin, xout, b, threads, s, r0, bout, uout); return 0;
+#define OW3D_NZ(N) case N: return rb_launch_nz<T, N>(L, xin, xout, b, threads, s, r0, bout, uout, pro) ? 0 : 1;
         OW3D_NZ(3) OW3D_NZ(4) OW3D_NZ(5) OW3D_NZ(6) OW3D_NZ(7) OW3D_NZ(8) OW3D_NZ(9) OW3D_NZ(10)
         OW3D_NZ(11) OW3D_NZ(12) OW3D_NZ(13) OW3D_NZ(14)
 #undef OW3D_NZ
@@ -502,15 +523,18 @@ int rb_launch(const LvArgs<T> &L, const T *xin, T *xout, const T *b, int threads
 // r0 (fused sweeps, nu >= 1 only): the level's x is zero and its b is (T) r0, both still to be set; the
 // first sweep does it.  uout (fused sweeps, nu >= 1 only): the last sweep writes its result there in FP64
 // (cur is then stale).
+// pro (fused sweeps, nu >= 1 only): the first sweep adds the prolongated coarse correction to cur's values;
+// returns 1 with nothing launched when that sweep form does not fit.
 template <typename T>
 int smooth(ow3d_tank *t, MgLevel<T> &L, int nu, T *&cur, hipStream_t s, const double *r0 = nullptr,
-           double *uout = nullptr) {
+           double *uout = nullptr, const ProArgs<T> *pro = nullptr) {
     if (t->fused_rb) {   // each launch is (ghost pass, red, black, ghost pass)
         if (nu == 0) ghost_pass(L, cur, s);
         for (int it = 0; it < nu; it++) {
             T *nxt = (cur == L.x.p) ? L.x2.p : L.x.p;
-            if (rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s, it == 0 ? r0 : nullptr, L.b.p,
-                             it == nu - 1 ? uout : nullptr)) return -1;
+            const int rc = rb_launch<T>(L.args, cur, nxt, L.b.p, t->rb_threads, s, it == 0 ? r0 : nullptr, L.b.p,
+                                        it == nu - 1 ? uout : nullptr, it == 0 ? pro : nullptr);
+            if (rc) return rc;
             cur = nxt;
         }
         return 0;
@@ -566,6 +590,14 @@ int vcycle(ow3d_tank *t, std::vector<std::unique_ptr<MgLevel<T>>> &mg, int k, T 
     }
     T *ccur = C.x.p;
     if (vcycle(t, mg, k - 1, ccur, s)) return -1;
+    // with fused sweeps the first post-smoothing sweep adds the prolongated correction itself; FP64 levels
+    // above 2^20 points keep the separate prolongation (at 515^2 x 10 the coarse window takes the FP64
+    // sweep tile from 50 to 57 KiB of LDS, two blocks per CU instead of three: 0.55 ms per RK4 step slower)
+    if (L.ptab && t->fused_rb && t->cfg.nu2 >= 1 && L.args.dim3 && (sizeof(T) == 4 || L.args.spec)) {
+        const ProArgs<T> P{(const T *)ccur, L.pxi.p, L.pyi.p, L.pxw.p, L.pyw.p, C.g.nxg};
+        const int rc = smooth(t, L, t->cfg.nu2, cur, s, nullptr, uout, &P);
+        if (rc <= 0) return rc;
+    }
     if (L.ptab) prolong_tab_launch<T>(G, L, (const T *)ccur, cur, s);
     else prolong_launch<T>(G, (const T *)ccur, cur, s);
     if (smooth(t, L, t->cfg.nu2, cur, s, nullptr, uout)) return -1;
