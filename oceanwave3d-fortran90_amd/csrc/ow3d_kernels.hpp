@@ -1511,19 +1511,29 @@ __device__ inline void solve_stats(KrylovState *S) {
 // End of a GMRES solve: back substitution H y = g on the kk leading columns (trailing zero pivots
 // dropped), y to S->y; the solution update follows in k_basis_combine.  slots < full: the solve was
 // enqueued with fewer Arnoldi steps than its budget (flags overflow when it used them all).
-static __global__ void k_gmres_finish(KrylovState *S, int slots, int full, int stage) {
-    if (threadIdx.x != 0) return;
-    if (!S->done && slots < full) S->overflow = 1;
-    if (S->k > S->amax[stage]) S->amax[stage] = S->k;
+// (One wave stages the leading rows of H and g in LDS; lane 0 then substitutes.)
+static __global__ __launch_bounds__(64) void k_gmres_finish(KrylovState *S, int slots, int full, int stage) {
     constexpr int M = KRYLOV_MAXK;
+    __shared__ double sH[M * M], sg[M], sy[M];
+    const int t = threadIdx.x, k = S->k;
+    const bool solve = !(S->beta <= S->tol);
+    if (solve) {
+        for (int e = t; e < k * M; e += 64) sH[e] = S->H[e];
+        if (t < k) sg[t] = S->g[t];
+    }
+    __syncthreads();
+    if (t != 0) return;
+    if (!S->done && slots < full) S->overflow = 1;
+    if (k > S->amax[stage]) S->amax[stage] = k;
     int kk = 0;
-    if (!(S->beta <= S->tol)) {
-        kk = S->k;
-        while (kk > 0 && S->H[(kk - 1) * M + (kk - 1)] == 0.0) kk--;
+    if (solve) {
+        kk = k;
+        while (kk > 0 && sH[(kk - 1) * M + (kk - 1)] == 0.0) kk--;
         for (int i = kk - 1; i >= 0; i--) {
-            double acc = S->g[i];
-            for (int q = i + 1; q < kk; q++) acc -= S->H[i * M + q] * S->y[q];
-            S->y[i] = acc / S->H[i * M + i];
+            double acc = sg[i];
+            for (int q = i + 1; q < kk; q++) acc -= sH[i * M + q] * sy[q];
+            sy[i] = acc / sH[i * M + i];
+            S->y[i] = sy[i];
         }
         S->status = (S->est <= S->tol) ? 0 : 1;
     }
