@@ -173,6 +173,18 @@ def test_precond_fp64_parity(oracle_mod, nx, ny, nz, shoal):
     assert rel(ug, uo) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_precond_at_513_matches_the_oracle(oracle_mod, prec):
+    """513 x 513 x 9 shoal tank, full hierarchy: the largest level runs the launch forms chosen for levels
+    above 2^20 points (FP64: 16 x 4 sweep tiles of 256 threads and a separate prolongation; FP32: 16 x 8
+    sweep tiles of 512 threads with the prolongation inside the post-smoothing sweep, 32 x 4 tiles of 512
+    threads on the 257^2 level), the residual and restriction in one launch on every level.  One V cycle
+    against the oracle's."""
+    T_o, T_g, _ = make_pair(oracle_mod, 513, 513, 9, shoal=True, L=50.0, mg_precision=prec)
+    r = np.random.default_rng(21).standard_normal(T_o.shape3)
+    assert rel(T_g.precond_apply(r), T_o.precond(r)) < (1e-10 if prec == 0 else 1e-4)
+
+
 def test_precond_with_a_large_coarsest_level(oracle_mod):
     """Two levels: the coarsest is 17 x 17 x 9 (3,610 unknowns), inverted on the host by banded LU on
     several threads; the V cycle must match the oracle's (dense LU coarse solve)."""
