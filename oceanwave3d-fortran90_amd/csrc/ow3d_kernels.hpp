@@ -2332,6 +2332,8 @@ __global__ __launch_bounds__(NT) void k_mg_zebra_rb(LvArgs<T> L, const T *__rest
     // values are computed from
     const int nxi = nxg - 2 * L.gx, nyi = d3 ? nyg - 2 * L.gy : 1;
     const int offx = (nxi % TX == 1) ? 1 : 0, offy = (d3 && nyi % TY == 1) ? 1 : 0;
+    // (tiles in launch order: the XCD-aware order of k_mg_residual_restrict halves the sweep's HBM fetch at
+    // 513^2, 55 -> 29 MB, but the sweep then takes 49.4 instead of 46.0 us)
     const int i0 = L.gx + cmax(0, (int)blockIdx.x * TX - offx), ie = L.gx + cmin(nxi, ((int)blockIdx.x + 1) * TX - offx);
     const int j0 = d3 ? L.gy + cmax(0, (int)blockIdx.y * TY - offy) : 0;
     const int je = d3 ? L.gy + cmin(nyi, ((int)blockIdx.y + 1) * TY - offy) : 1;
@@ -2830,7 +2832,11 @@ __global__ __launch_bounds__(NT) void k_mg_residual_restrict(LvArgs<T> L, Restri
     __shared__ T sC[S::C];
     __shared__ T sW[S::W];
     const int nxg = L.nxg, nyg = L.nyg, t = (int)threadIdx.x;
-    const int ic0 = (int)blockIdx.x * CX, jc0 = (int)blockIdx.y * CY;
+    // XCD-aware tile order (xcd_tile over the row-major tile index): each XCD takes one contiguous band of
+    // tile rows, so the halos neighbouring tiles share are re-read from that XCD's L2 (at 513^2: HBM fetch
+    // 57 -> 29 MB, 22.2 -> 20.0 us)
+    const int tb = xcd_tile((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+    const int ic0 = (tb % (int)gridDim.x) * CX, jc0 = (tb / (int)gridDim.x) * CY;
     // fine column of coarse column ic is 2 ic - 1; the tile's stencils reach fine columns fi0 .. fi0 + FW - 1
     const int fi0 = 2 * ic0 - 2, fj0 = 2 * jc0 - 2;
     const int lox = cmax(cmin(fi0 - R, nxg - 4), 0), loy = cmax(cmin(fj0 - R, nyg - 4), 0);
