@@ -242,15 +242,34 @@ PlaneArgs plane_args(ow3d_tank *t, double *E, double *P) {
 
 void surface_dev(ow3d_tank *t, double *E, double *P, hipStream_t s) {
     PlaneArgs a = plane_args(t, E, P);
-    hipLaunchKernelGGL(k_surface_x, dim3(t->g.nyg), dim3(256), 0, s, a);
-    if (t->g.ny > 1) hipLaunchKernelGGL(k_surface_y, dim3((t->g.nxg + 63) / 64, (t->g.nyg + 3) / 4), dim3(256), 0, s, a);
+    // stencil widths of 4th, 6th and 8th order as template arguments, others at run time
+    switch (2 * t->g.a + 1) {
+        case 5: hipLaunchKernelGGL((k_surface_x<5>), dim3(t->g.nyg), dim3(256), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_surface_x<7>), dim3(t->g.nyg), dim3(256), 0, s, a); break;
+        case 9: hipLaunchKernelGGL((k_surface_x<9>), dim3(t->g.nyg), dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_surface_x<0>), dim3(t->g.nyg), dim3(256), 0, s, a); break;
+    }
+    if (t->g.ny > 1) {
+        const dim3 grid((t->g.nxg + 63) / 64, (t->g.nyg + 3) / 4);
+        switch (2 * t->g.b + 1) {
+            case 5: hipLaunchKernelGGL((k_surface_y<5>), grid, dim3(256), 0, s, a); break;
+            case 7: hipLaunchKernelGGL((k_surface_y<7>), grid, dim3(256), 0, s, a); break;
+            case 9: hipLaunchKernelGGL((k_surface_y<9>), grid, dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_surface_y<0>), grid, dim3(256), 0, s, a); break;
+        }
+    }
 }
 
 void wvel_dev(ow3d_tank *t, const double *phi, double *W, hipStream_t s) {
     const int rc = 2 * t->g.c + 1;
     const double *szTop = t->sz.p + (size_t)((t->g.nzg - 1) * 2) * rc;
-    hipLaunchKernelGGL(k_vertical_velocity, dim3((t->g.nxy + 255) / 256), dim3(256), 0, s, t->g.nxy, t->g.nzg, rc, szTop,
-                       (const double *)t->cf.p, phi, W);
+    const dim3 grid((t->g.nxy + 255) / 256);
+    switch (rc) {
+        case 5: hipLaunchKernelGGL((k_vertical_velocity<5>), grid, dim3(256), 0, s, t->g.nxy, t->g.nzg, rc, szTop, (const double *)t->cf.p, phi, W); break;
+        case 7: hipLaunchKernelGGL((k_vertical_velocity<7>), grid, dim3(256), 0, s, t->g.nxy, t->g.nzg, rc, szTop, (const double *)t->cf.p, phi, W); break;
+        case 9: hipLaunchKernelGGL((k_vertical_velocity<9>), grid, dim3(256), 0, s, t->g.nxy, t->g.nzg, rc, szTop, (const double *)t->cf.p, phi, W); break;
+        default: hipLaunchKernelGGL((k_vertical_velocity<0>), grid, dim3(256), 0, s, t->g.nxy, t->g.nzg, rc, szTop, (const double *)t->cf.p, phi, W); break;
+    }
 }
 
 }  // namespace

@@ -810,20 +810,24 @@ struct PlaneArgs {
 };
 
 // one block per row j: x ghost values of E and P (rows with j in [gy, nyg-gy)), then x derivatives.
+// RA: the stencil width 2a + 1 when known at compile time (loads of all taps issued together), 0: g.a's.
+template <int RA>
 static __global__ __launch_bounds__(256) void k_surface_x(PlaneArgs g) {
-    const int j = blockIdx.x, nxg = g.nxg, ra = 2 * g.a + 1;
+    const int j = blockIdx.x, nxg = g.nxg, ra = RA > 0 ? RA : 2 * g.a + 1;
     double *E = g.E + (size_t)j * nxg, *P = g.P + (size_t)j * nxg;
     if (j >= g.gy && j < g.nyg - g.gy && threadIdx.x < 4) {
         double *v = (threadIdx.x & 1) ? P : E;
         if (threadIdx.x < 2) {
             const double *st = g.sx + (size_t)(1 * 2) * ra;
             double s = 0.0;
-            for (int m = 1; m < ra; m++) s += st[m] * v[m];
+#pragma unroll
+            for (int m = 1; m < (RA > 0 ? RA : ra); m++) s += st[m] * v[m];
             v[0] = (0.0 - s) / st[0];
         } else {
             const double *st = g.sx + (size_t)((nxg - 2) * 2) * ra;
             double s = 0.0;
-            for (int m = 0; m < ra - 1; m++) s += st[m] * v[nxg - ra + m];
+#pragma unroll
+            for (int m = 0; m < (RA > 0 ? RA : ra) - 1; m++) s += st[m] * v[nxg - ra + m];
             v[nxg - 1] = (0.0 - s) / st[ra - 1];
         }
     }
@@ -832,7 +836,8 @@ static __global__ __launch_bounds__(256) void k_surface_x(PlaneArgs g) {
         const int ws = cclamp(i - g.a, 0, nxg - ra);
         const double *S1 = g.sx + (size_t)(i * 2) * ra, *S2 = S1 + ra;
         double e1 = 0.0, e2 = 0.0, p1 = 0.0;
-        for (int m = 0; m < ra; m++) { e1 += S1[m] * E[ws + m]; e2 += S2[m] * E[ws + m]; p1 += S1[m] * P[ws + m]; }
+#pragma unroll
+        for (int m = 0; m < (RA > 0 ? RA : ra); m++) { e1 += S1[m] * E[ws + m]; e2 += S2[m] * E[ws + m]; p1 += S1[m] * P[ws + m]; }
         const size_t q = (size_t)j * nxg + i;
         g.Ex[q] = e1; g.Exx[q] = e2; g.Px[q] = p1;
         if (g.nyg == 1) {  // x-z problem: coefficient fields complete here
@@ -846,8 +851,9 @@ static __global__ __launch_bounds__(256) void k_surface_x(PlaneArgs g) {
 // point, 64 consecutive columns x 4 rows per block so that every load and store of a wave is contiguous.
 // A ghost value depends only on interior values of its column, so each thread whose stencil reaches a
 // ghost row evaluates it itself; the threads of the ghost rows also write it back.
+template <int RB>
 static __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
-    const int nxg = g.nxg, nyg = g.nyg, rb = 2 * g.b + 1;
+    const int nxg = g.nxg, nyg = g.nyg, rb = RB > 0 ? RB : 2 * g.b + 1;
     const int i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), j = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
     if (i >= nxg || j >= nyg) return;
     const bool gcol = i >= g.gx && i < nxg - g.gx;
@@ -855,13 +861,15 @@ static __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
         if (gcol && jj == 0) {
             const double *st = g.sy + (size_t)(1 * 2) * rb;
             double s = 0.0;
-            for (int m = 1; m < rb; m++) s += st[m] * v[i + (size_t)nxg * m];
+#pragma unroll
+            for (int m = 1; m < (RB > 0 ? RB : rb); m++) s += st[m] * v[i + (size_t)nxg * m];
             return (0.0 - s) / st[0];
         }
         if (gcol && jj == nyg - 1) {
             const double *st = g.sy + (size_t)((nyg - 2) * 2) * rb;
             double s = 0.0;
-            for (int m = 0; m < rb - 1; m++) s += st[m] * v[i + (size_t)nxg * (nyg - rb + m)];
+#pragma unroll
+            for (int m = 0; m < (RB > 0 ? RB : rb) - 1; m++) s += st[m] * v[i + (size_t)nxg * (nyg - rb + m)];
             return (0.0 - s) / st[rb - 1];
         }
         return v[i + (size_t)nxg * jj];
@@ -869,7 +877,8 @@ static __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
     const int ws = cclamp(j - g.b, 0, nyg - rb);
     const double *S1 = g.sy + (size_t)(j * 2) * rb, *S2 = S1 + rb;
     double e1 = 0.0, e2 = 0.0, p1 = 0.0;
-    for (int m = 0; m < rb; m++) {
+#pragma unroll
+    for (int m = 0; m < (RB > 0 ? RB : rb); m++) {
         const double ev = val(g.E, ws + m), pv = val(g.P, ws + m);
         e1 += S1[m] * ev; e2 += S2[m] * ev; p1 += S1[m] * pv;
     }
@@ -887,13 +896,17 @@ static __global__ __launch_bounds__(256) void k_surface_y(PlaneArgs g) {
 }
 
 // W = dsigma/dz * one-sided sigma derivative at the surface (VerticalFreeSurfaceVelocity.f90:17).
+// RC: the stencil width when known at compile time, 0: rc.
+template <int RC>
 static __global__ void k_vertical_velocity(int nxy, int nzg, int rc, const double *__restrict__ szTop,
                                     const double *__restrict__ cf, const double *__restrict__ phi, double *__restrict__ W) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nxy) return;
+    if (RC > 0) rc = RC;
     const double *col = phi + (size_t)p * nzg + (nzg - rc);
     double s = 0.0;
-    for (int m = 0; m < rc; m++) s += szTop[m] * col[m];
+#pragma unroll
+    for (int m = 0; m < (RC > 0 ? RC : rc); m++) s += szTop[m] * col[m];
     W[p] = cf[6 * nxy + p] * s;   // dsigma/dz = 1/d
 }
 
